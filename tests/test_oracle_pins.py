@@ -1,0 +1,312 @@
+"""Pins for the CPU restatement (the oracle).
+
+The reference ships no tests, decks or golden outputs and cannot be built here
+(it needs MPI and its whole 184-file program), so every oracle function is
+pinned against mathematics that a plausible mistake would break: closed forms,
+brute force over explicit periodic images, finite differences, invariants and
+the ncell = 2 limit in which a CAC element is exactly eight atoms.
+"""
+
+import numpy as np
+import pytest
+
+from cac_amd import decks
+from oracle import oracle as O
+from tests import brute
+
+O.build()
+TAB = decks.cu_eam_tables()
+
+
+# ---------------------------------------------------------------- splines (pair_eam.cpp:586-614)
+
+def test_spline_reproduces_cubic():
+    """The 5-point derivative is exact for cubics and the Hermite segment then
+    reproduces the cubic; a wrong coefficient or sign breaks this."""
+    n, dx = 60, 0.05
+    xs = np.arange(n) * dx  # knot m (1-based) at (m-1)*dx
+
+    def f(x):
+        return 0.3 - 1.1 * x + 0.7 * x ** 2 - 0.9 * x ** 3
+
+    def fp(x):
+        return -1.1 + 1.4 * x - 2.7 * x ** 2
+
+    spl = O.interpolate(np.concatenate([[0.0], f(xs)]), dx)
+    for m in range(3, n - 3):
+        for p in (0.0, 0.25, 0.5, 0.9):
+            x = (m - 1 + p) * dx
+            val, der = brute.spline_eval(spl, m, p)
+            assert abs(val - f(x)) < 1e-12
+            assert abs(der - fp(x)) < 1e-10
+    # end knots use the 2-point derivative, and the last knot has no cubic term
+    assert spl[n, 3] == 0.0 and spl[n, 4] == 0.0
+    assert spl[1, 5] == pytest.approx(f(xs[1]) - f(xs[0]), abs=1e-15)
+
+
+def test_spline_quartic_derivative_exact():
+    n, dx = 40, 0.1
+    xs = np.arange(n) * dx
+    spl = O.interpolate(np.concatenate([[0.0], xs ** 4]), dx)
+    for m in range(3, n - 1):
+        assert spl[m, 5] == pytest.approx(4 * xs[m - 1] ** 3 * dx, rel=1e-12, abs=1e-13)
+
+
+# ---------------------------------------------------------------- element tables
+
+@pytest.mark.parametrize("nc", [2, 3, 9, 11])
+def test_element_tables_closed_forms(nc):
+    t = O.etype_tables((nc, nc, nc))
+    nu = nc ** 3
+    assert t["nucell"] == nu and t["ngcell"] == 8
+    corners = brute.corner_ucells((nc, nc, nc))
+    assert list(t["g2u"]) == list(corners)
+    # nodes follow from N = 1: Hex natural signs vs the i-major corner order
+    assert list(t["g2n"]) == [0, 4, 3, 7, 1, 5, 2, 6]
+    assert all(t["g2n"][t["n2g"][k]] == k for k in range(8))
+    assert t["weight"].sum() == pytest.approx(nu)
+    assert np.allclose(t["weight"], nu / 8.0)
+    assert t["nodal_weight"] == nu / 8.0
+    shape = t["shape"]
+    assert np.allclose(shape.sum(axis=1), 1.0, atol=1e-15)
+    assert np.allclose(shape, brute.shape_hex(brute.ucell_natural((nc, nc, nc))), atol=1e-15)
+    for g, u in enumerate(t["g2u"]):
+        e = np.zeros(8)
+        e[t["g2n"][g]] = 1.0
+        assert np.array_equal(shape[u], e)
+    u2g = t["u2g"]
+    assert (u2g >= 0).sum() == 8
+    # CONSISTENT weighted shape: N * w / nucell = delta / 8 at the nodes
+    assert np.allclose(t["wsa"], np.eye(8)[t["g2n"]] / 8.0)
+    # sub-elements partition the unit cells exactly once
+    allu = np.concatenate([t["us2u"][s, :t["nucell_sub"][s]] for s in range(t["nsub"])])
+    assert np.array_equal(np.sort(allu), np.arange(nu))
+    es = int(round(np.sqrt(np.cbrt(nu))))
+    assert t["nsub"] == es ** 3
+
+
+def test_mass_matrices():
+    m, mi = O.mass_matrices()
+    assert np.allclose(m @ mi, np.eye(8), atol=1e-13)
+    assert m.sum() == pytest.approx(1.0)
+    assert set(np.unique(mi)) == {64.0, -32.0, 16.0, -8.0}
+
+
+# ---------------------------------------------------------------- LJ vs brute force
+
+def _lj_deck(n=5, jitter=0.05, seed=1):
+    return decks.lj_block(n=n, jitter=jitter, T=0.1, seed=seed)
+
+
+def _lj_energy(x, L, cut=2.5):
+    d = x[:, None, :] - x[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = np.einsum("ijk,ijk->ij", d, d)
+    np.fill_diagonal(r2, np.inf)
+    m = r2 < cut * cut
+    r6 = np.where(m, 1.0 / np.where(m, r2, 1.0) ** 3, 0.0)
+    return 0.5 * np.sum(4.0 * (r6 * r6 - r6))
+
+
+def test_lj_energy_forces_brute_force():
+    d = _lj_deck()
+    o = O.Oracle(d, "lj", skin=0.3, delay=0)
+    o.setup()
+    x, _, f = o.atoms()
+    L = d.boxhi - d.boxlo
+    e, vir = o.energy()
+    assert e == pytest.approx(_lj_energy(x, L), rel=1e-12)
+    assert np.abs(f.sum(axis=0)).max() < 1e-10
+    h = 1e-6
+    for i in (0, 77, 311):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            fd = -(_lj_energy(xp, L) - _lj_energy(xm, L)) / (2 * h)
+            assert f[i, k] == pytest.approx(fd, rel=1e-6, abs=1e-6)
+    # pair virial sum_{i<j} r_ij (x) f_ij equals sum_i x_i . f_i for a periodic
+    # system only up to image terms; check trace against the direct pair sum
+    dd = x[:, None, :] - x[None, :, :]
+    dd -= L * np.round(dd / L)
+    r2 = np.einsum("ijk,ijk->ij", dd, dd)
+    np.fill_diagonal(r2, np.inf)
+    msk = r2 < 6.25
+    r2i = np.where(msk, 1.0 / np.where(msk, r2, 1.0), 0.0)
+    r6 = r2i ** 3
+    fpair = r6 * (48 * r6 - 24) * r2i
+    vxx = 0.5 * np.sum(dd[..., 0] ** 2 * fpair)
+    assert vir[0] == pytest.approx(vxx, rel=1e-11)
+
+
+def test_nve_time_reversible():
+    d = _lj_deck(jitter=0.02)
+    o = O.Oracle(d, "lj", skin=0.3, delay=0)
+    o.setup()
+    x0, v0, _ = o.atoms()
+    o.run(50, 0.005)
+    x1, v1, _ = o.atoms()
+    d2 = decks.Deck(units="lj", boxlo=d.boxlo, boxhi=d.boxhi, masses=d.masses, x=x1, v=-v1, atype=d.atype)
+    o2 = O.Oracle(d2, "lj", skin=0.3, delay=0)
+    o2.setup()
+    o2.run(50, 0.005)
+    x2, v2, _ = o2.atoms()
+    L = d.boxhi - d.boxlo
+    dx = x2 - x0
+    dx -= L * np.round(dx / L)
+    assert np.abs(dx).max() < 1e-9
+    assert np.abs(v2 + v0).max() < 1e-9
+
+
+def test_nve_energy_error_scales_dt2():
+    """Velocity Verlet's energy error falls as dt^2 (smooth tapered EAM; the
+    unshifted LJ cutoff would add dt-independent jumps)."""
+    d = _eam_atoms_deck(n=6, jitter=0.05)
+    rng = np.random.default_rng(3)
+    d.v = rng.normal(0.0, 2.0, d.x.shape)
+    errs = []
+    for dt in (0.002, 0.001):
+        o = O.Oracle(d, "eam", skin=2.0, eam_tables=TAB)
+        o.setup()
+        th = o.run(int(round(0.2 / dt)), dt, thermo_every=int(round(0.01 / dt)))
+        errs.append(np.abs(th[:, 3] - th[0, 3]).max())
+    assert 3.0 < errs[0] / errs[1] < 5.5
+
+
+# ---------------------------------------------------------------- EAM, pure atoms
+
+def _eam_atoms_deck(n=6, jitter=0.05, seed=7):
+    a = 3.615
+    rng = np.random.default_rng(seed)
+    x = decks.fcc_sites(a, (n, n, n)) + rng.normal(0, jitter, (4 * n ** 3, 3))
+    L = n * a
+    x = np.mod(x, L)
+    return decks.Deck(units="metal", boxlo=np.zeros(3), boxhi=np.full(3, L), masses=np.array([63.546]), x=x,
+                      v=np.zeros_like(x), atype=np.ones(len(x), dtype=np.int32))
+
+
+def test_eam_atoms_brute_force_and_fd():
+    d = _eam_atoms_deck()
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB)
+    o.setup()
+    x, _, f = o.atoms()
+    L = d.boxhi - d.boxlo
+    e, _ = o.energy()
+    assert e == pytest.approx(brute.eam_atoms_energy(x, L, TAB), rel=1e-12)
+    # the splines track the analytic functions they tabulate
+    assert e == pytest.approx(brute.eam_atoms_energy(x, L, TAB, analytic=True), rel=1e-6)
+    assert np.abs(f.sum(axis=0)).max() < 1e-9
+    h = 1e-5
+    for i in (3, 500):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            fd = -(brute.eam_atoms_energy(xp, L, TAB) - brute.eam_atoms_energy(xm, L, TAB)) / (2 * h)
+            assert f[i, k] == pytest.approx(fd, rel=1e-5, abs=1e-5)
+
+
+# ---------------------------------------------------------------- CAC closed forms
+
+def _perfect_energy_per_site(a=3.615, strain=(0.0, 0.0, 0.0)):
+    x = decks.fcc_sites(a, (6, 6, 6)) * (1.0 + np.asarray(strain))
+    L = 6 * a * (1.0 + np.asarray(strain))
+    return brute.eam_atoms_energy(x, L, TAB) / x.shape[0]
+
+
+@pytest.mark.parametrize("nc,box,core", [(2, (4, 4, 4), 20), (3, (3, 3, 3), 8), (5, (2, 2, 2), 4), (9, (2, 2, 2), 4)])
+def test_cac_perfect_lattice_energy_and_zero_force(nc, box, core):
+    """A perfect lattice represented by atoms and elements has the energy of
+    every represented site at the cohesive energy (nodal weights, energy scales
+    and list completeness) and no force anywhere."""
+    d = decks.cac_eam(box_cells=box, core_cells=core, ncell=nc, jitter=0.0, T=0.0, jitter_atoms=False)
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB)
+    o.setup()
+    e, _ = o.energy()
+    assert e / d.represented_sites == pytest.approx(_perfect_energy_per_site(), rel=1e-12)
+    _, _, f = o.atoms()
+    el = o.elements()
+    assert np.abs(f).max() < 1e-11
+    assert np.abs(el["gaussf"]).max() < 1e-11
+    assert np.abs(el["nodef"]).max() < 1e-10
+
+
+def test_cac_affine_strain_zero_force():
+    strain = np.array([0.01, -0.005, 0.0])
+    d = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.0, T=0.0, jitter_atoms=False)
+    F = 1.0 + strain
+    d.x = d.x * F
+    d.nodex = d.nodex * F
+    d.boxhi = d.boxhi * F
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB)
+    o.setup()
+    e, _ = o.energy()
+    assert e / d.represented_sites == pytest.approx(_perfect_energy_per_site(strain=strain), rel=1e-12)
+    assert np.abs(o.elements()["gaussf"]).max() < 1e-11
+
+
+def test_cac_ncell2_is_atomistic():
+    """With 2x2x2 sites per element every site is a node and a Gauss point, so
+    the CAC Gauss forces, densities and energy equal the all-atom ones."""
+    d = decks.cac_eam(box_cells=(4, 4, 4), core_cells=20, ncell=2, jitter=0.04, T=0.0, seed=11)
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB)
+    o.setup()
+    e_cac, vir_cac = o.energy()
+    _, _, fa = o.atoms()
+    el = o.elements()
+    ar, nr = o.rho()
+    allx = np.concatenate([d.x, d.nodex.reshape(-1, 3)], axis=0)
+    da = decks.Deck(units="metal", boxlo=d.boxlo, boxhi=d.boxhi, masses=d.masses, x=np.mod(allx, d.boxhi),
+                    v=np.zeros_like(allx), atype=np.ones(len(allx), dtype=np.int32))
+    oa = O.Oracle(da, "eam", skin=1.0, eam_tables=TAB)
+    oa.setup()
+    e_at, vir_at = oa.energy()
+    _, _, f_at = oa.atoms()
+    rho_at, _ = oa.rho()
+    A = d.natoms
+    assert e_cac == pytest.approx(e_at, rel=1e-12)
+    assert np.allclose(vir_cac, vir_at, rtol=1e-11, atol=1e-9)
+    assert np.allclose(fa, f_at[:A], atol=1e-10)
+    # Gauss cell g sits at node g2n[g]; the all-atom index of node k of element e is A + 8e + k
+    g2n = np.array([0, 4, 3, 7, 1, 5, 2, 6])
+    fnode = f_at[A:].reshape(-1, 8, 3)
+    assert np.allclose(el["gaussf"], fnode[:, g2n, :], atol=1e-10)
+    assert np.allclose(ar, rho_at[:A], rtol=1e-13)
+    assert np.allclose(nr, rho_at[A:].reshape(-1, 8), rtol=1e-13)
+    # consistent nodal assembly: 8 M nodef = gaussf at the nodes (element.cpp:2178-2211)
+    m, _ = O.mass_matrices()
+    gf_node = np.zeros_like(el["gaussf"])
+    gf_node[:, g2n, :] = el["gaussf"]
+    assert np.allclose(8.0 * np.einsum("ab,ebd->ead", m, el["nodef"]), gf_node, atol=1e-9)
+
+
+# ---------------------------------------------------------------- neighbor sets
+
+@pytest.mark.parametrize("nc,box,core,skin", [(3, (3, 3, 3), 8, 1.0), (5, (2, 2, 2), 4, 2.0)])
+def test_neighbor_sets_match_brute_force(nc, box, core, skin):
+    """Half/bin/newton lists over atoms, Gauss points and virtual atoms, ghosts
+    and sub-element pruning expand to exactly the brute-force set of
+    (point, site, image) pairs within rc + skin."""
+    d = decks.cac_eam(box_cells=box, core_cells=core, ncell=nc, jitter=0.05, T=0.0, seed=nc)
+    o = O.Oracle(d, "eam", skin=skin, eam_tables=TAB)
+    o.build_neighbor()
+    p, k, ow, u, img = o.half_list()
+    g2u = brute.corner_ucells(d.ncell)
+    got = O.canonical_full_sets(d, p, k, ow, u, img, g2u)
+    want = brute.full_sets(d, TAB.cut + skin)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+    # every unordered point-point pair is stored exactly once
+    pts = k == 0
+    assert len(np.unique(np.stack([p[pts], ow[pts], img[pts, 0], img[pts, 1], img[pts, 2]], 1), axis=0)) == pts.sum()
+
+
+def test_cac_force_is_deterministic():
+    d = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=300.0, seed=2)
+    res = []
+    for _ in range(2):
+        o = O.Oracle(d, "eam", skin=2.0, eam_tables=TAB)
+        o.setup()
+        o.run(20, 0.001)
+        res.append(o.elements()["nodex"])
+    assert np.array_equal(res[0], res[1])
