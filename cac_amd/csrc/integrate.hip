@@ -1,0 +1,248 @@
+// integrate.hip -- velocity Verlet, element centres, nodal force assembly and
+// deterministic reductions.
+//
+// Replaces FixNVE::initial_integrate / final_integrate (fix_nve.cpp:62-217),
+// ElementVecCAC::update_center_coord (element_vec_cac.cpp:3740-3765),
+// Element::compute_nodef (element.cpp:2178-2246) and the thermo sums of
+// ComputeTemp / ComputePE / ComputePressure (compute_temp.cpp:73-157).
+#include "cac_ctx.hpp"
+
+namespace cac {
+
+struct MassTab {
+  double inv[MAXT + 1];  // dtf / mass[type] is formed per point as the reference does
+  double m[MAXT + 1];
+};
+
+// atoms and nodes move the same way; nodes use mass[ctype] (fix_nve.cpp:118-131)
+__global__ void k_nve_initial(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ f,
+                              const int* __restrict__ ptype, int P, MassTab mt, double dtf, double dtv) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double dtfm = dtf / mt.m[ptype[p]];
+  for (int d = 0; d < 3; d++) {
+    const size_t i = (size_t)d * P + p;
+    const double vv = v[i] + dtfm * f[i];
+    v[i] = vv;
+    x[i] = x[i] + dtv * vv;
+  }
+}
+
+__global__ void k_nve_final(double* __restrict__ v, const double* __restrict__ f, const int* __restrict__ ptype,
+                            int P, MassTab mt, double dtf) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double dtfm = dtf / mt.m[ptype[p]];
+  for (int d = 0; d < 3; d++) {
+    const size_t i = (size_t)d * P + p;
+    v[i] = v[i] + dtfm * f[i];
+  }
+}
+
+// element centre = mean of its nodes, summed in node order then / npe / apc
+__global__ void k_centers(const double* __restrict__ x, double* __restrict__ ex, int A, int E, int P) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  for (int d = 0; d < 3; d++) {
+    const double* nx = x + (size_t)d * P + A + (size_t)NPE * e;
+    double s = 0.0;
+    for (int k = 0; k < NPE; k++) s += nx[k];
+    ex[(size_t)d * E + e] = s / NPE / 1;
+  }
+}
+
+static MassTab mass_tab(const Ctx& c) {
+  MassTab t;
+  for (int i = 0; i <= MAXT; i++) {
+    t.m[i] = c.mass[i] > 0 ? c.mass[i] : 1.0;
+    t.inv[i] = 1.0 / t.m[i];
+  }
+  return t;
+}
+
+void nve_initial(Ctx& c, double dt) {
+  ScopedTimer tm(c, "nve");
+  const double dtf = 0.5 * dt * c.ftm2v;
+  k_nve_initial<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.v.get(), c.f.get(), c.ptype.get(), c.P,
+                                                          mass_tab(c), dtf, dt);
+  if (c.E) update_centers(c);
+}
+
+void nve_final(Ctx& c, double dt) {
+  ScopedTimer tm(c, "nve");
+  const double dtf = 0.5 * dt * c.ftm2v;
+  k_nve_final<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.v.get(), c.f.get(), c.ptype.get(), c.P, mass_tab(c), dtf);
+}
+
+void update_centers(Ctx& c) {
+  if (!c.E) return;
+  k_centers<<<grid_for(c.E, 256), 256, 0, c.stream>>>(c.x.get(), c.ex.get(), c.A, c.E, c.P);
+}
+
+// ------------------------------------------------------------------ nodal assembly
+// force_columns[k] = sum_g wsa[g][k] gaussf[g]; nodef[a] = sum_b Minv[a][b] fc[b]
+// (CONSISTENT), nodef = fc (LUMPED, with the lumped weights) or the Gauss-cell
+// force itself (NODEONLY).  One thread per element and dimension; no atomics.
+
+struct NodefTab {
+  double w[8][8];     // [g][node]
+  double minv[8][8];  // [a][b]
+  int g2n[8], n2g[8];
+  int style;
+};
+
+__global__ void k_nodef(const double* __restrict__ gf, double* __restrict__ f, int A, int E, int P, NodefTab t) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 3 * E) return;
+  const int d = id / E, e = id - d * E;
+  const int E8 = NPE * E;
+  // Gauss-cell forces of this element, indexed by Gauss cell g (stored at node g2n[g])
+  double g[8];
+  for (int q = 0; q < 8; q++) g[q] = gf[(size_t)d * E8 + (size_t)NPE * e + t.g2n[q]];
+  double* out = f + (size_t)d * P + A + (size_t)NPE * e;
+  if (t.style == 2) {
+    for (int k = 0; k < NPE; k++) out[k] = g[t.n2g[k]];
+    return;
+  }
+  double fc[8];
+  for (int k = 0; k < NPE; k++) {
+    double s = 0.0;
+    for (int q = 0; q < 8; q++) s += t.w[q][k] * g[q];
+    fc[k] = s;
+  }
+  if (t.style == 1) {
+    for (int k = 0; k < NPE; k++) out[k] = fc[k];
+    return;
+  }
+  for (int a = 0; a < NPE; a++) {
+    double s = 0.0;
+    for (int b = 0; b < NPE; b++) s += t.minv[a][b] * fc[b];
+    out[a] = s;
+  }
+}
+
+void nodal_assembly(Ctx& c) {
+  ScopedTimer tm(c, "nodef");
+  NodefTab t;
+  for (int g = 0; g < 8; g++)
+    for (int k = 0; k < 8; k++) {
+      t.w[g][k] = c.nodef_style == 1 ? c.wsa_lumped[g][k] : c.wsa[g][k];
+      t.minv[g][k] = c.minv[g][k];
+    }
+  for (int k = 0; k < 8; k++) {
+    t.g2n[k] = c.g2n[k];
+    t.n2g[k] = c.n2g[k];
+  }
+  t.style = c.nodef_style;
+  k_nodef<<<grid_for(3L * c.E, 256), 256, 0, c.stream>>>(c.gf.get(), c.f.get(), c.A, c.E, c.P, t);
+}
+
+// ------------------------------------------------------------------ deterministic reductions
+// Each block sums a fixed slice in a fixed tree; a single block then sums the
+// block partials in order.  No atomics: the result is bitwise reproducible.
+
+constexpr int RB = 256;
+
+__global__ __launch_bounds__(RB) void k_reduce_partial(const double* __restrict__ data, int n, int ncomp,
+                                                       double* __restrict__ partial) {
+  __shared__ double s[RB];
+  const int nb = gridDim.x;
+  for (int c = 0; c < ncomp; c++) {
+    double acc = 0.0;
+    for (long i = (long)blockIdx.x * RB + threadIdx.x; i < n; i += (long)nb * RB) acc += data[(size_t)c * n + i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = RB / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)c * nb + blockIdx.x] = s[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(RB) void k_reduce_final(const double* __restrict__ partial, int nb, int ncomp,
+                                                     double* __restrict__ out) {
+  __shared__ double s[RB];
+  for (int c = 0; c < ncomp; c++) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) acc += partial[(size_t)c * nb + i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = RB / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = s[0];
+    __syncthreads();
+  }
+}
+
+void reduce_rows(Ctx& c, const double* data, int n, int ncomp, double* out) {
+  const int nb = 256;
+  c.partials.reserve((size_t)nb * ncomp + 16);
+  double* fin = c.partials.get() + (size_t)nb * ncomp;
+  k_reduce_partial<<<nb, RB, 0, c.stream>>>(data, n, ncomp, c.partials.get());
+  k_reduce_final<<<1, RB, 0, c.stream>>>(c.partials.get(), nb, ncomp, fin);
+  CAC_HIP(hipMemcpyAsync(c.h_red, fin, sizeof(double) * ncomp, hipMemcpyDeviceToHost, c.stream));
+  CAC_HIP(hipStreamSynchronize(c.stream));
+  for (int i = 0; i < ncomp; i++) out[i] = c.h_red[i];
+}
+
+// ------------------------------------------------------------------ kinetic energy (compute_temp.cpp:96-150)
+
+struct KeTab {
+  double m[MAXT + 1];
+  double mm[8][8];
+  double nucell, nodal_weight;
+  int consistent;
+};
+
+__global__ void k_ke_rows(const double* __restrict__ v, const int* __restrict__ ptype, int A, int E, int P, KeTab t,
+                          double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = A + E;
+  if (i >= n) return;
+  if (i < A) {
+    const double vx = v[i], vy = v[(size_t)P + i], vz = v[2 * (size_t)P + i];
+    out[i] = t.m[ptype[i]] * (vx * vx + vy * vy + vz * vz);
+    return;
+  }
+  const int e = i - A;
+  const size_t b = (size_t)A + (size_t)NPE * e;
+  double sum = 0.0;
+  if (t.consistent) {
+    for (int k = 0; k < NPE; k++)
+      for (int l = 0; l < NPE; l++)
+        sum += t.mm[k][l] * (v[b + k] * v[b + l] + v[(size_t)P + b + k] * v[(size_t)P + b + l] +
+                             v[2 * (size_t)P + b + k] * v[2 * (size_t)P + b + l]);
+    out[i] = sum * t.m[ptype[b]] * t.nucell;
+  } else {
+    for (int k = 0; k < NPE; k++)
+      sum += v[b + k] * v[b + k] + v[(size_t)P + b + k] * v[(size_t)P + b + k] +
+             v[2 * (size_t)P + b + k] * v[2 * (size_t)P + b + k];
+    out[i] = sum * t.m[ptype[b]] * t.nodal_weight;
+  }
+}
+
+void thermo(Ctx& c, double* ke, double* temp) {
+  KeTab t;
+  for (int i = 0; i <= MAXT; i++) t.m[i] = c.mass[i];
+  for (int a = 0; a < 8; a++)
+    for (int b = 0; b < 8; b++) t.mm[a][b] = c.mmat[a][b];
+  t.nucell = c.nucell;
+  t.nodal_weight = c.nodal_weight;
+  t.consistent = c.nodef_style == 0;
+  const int n = c.A + c.E;
+  c.erow.reserve((size_t)7 * c.P);
+  k_ke_rows<<<grid_for(n, 256), 256, 0, c.stream>>>(c.v.get(), c.ptype.get(), c.A, c.E, c.P, t, c.erow.get());
+  double s = 0.0;
+  reduce_rows(c, c.erow.get(), n, 1, &s);
+  const double dof = 3.0 * ((double)c.A + (double)c.nucell * c.E);
+  const double tfactor = dof > 0 ? c.mvv2e / (dof * c.boltz) : 0.0;
+  const double T = s * tfactor;
+  if (temp) *temp = T;
+  if (ke) *ke = T * 0.5 * dof * c.boltz;
+}
+
+}  // namespace cac

@@ -1,0 +1,664 @@
+// neigh.hip -- periodic wrap, site table, counting-sort cell list and full
+// per-point neighbor lists over atoms, Gauss points and interpolated atoms.
+//
+// Replaces, on one GPU: Domain::pbc (domain.cpp:526-738), the deformation check
+// of Element::update_element_bound_box (element.cpp:1401-1465), the ghost shell of
+// CommBrick::borders (comm_brick.cpp:946-1310) and Neighbor::build with
+// NBinStandard / NStencilHalfBin3d / NPairHalfBinNewton (neighbor.cpp:1704-1873,
+// nbin_standard.cpp:338-370, npair_half_bin_newton.cpp:25-393).
+//
+// The reference builds half lists with reactions; here every point (atom or
+// Gauss point) gets the FULL set of sites (atoms, Gauss points and virtual atoms
+// of every element, with periodic images) within rc + skin, so the force kernels
+// gather and never scatter.  Periodic images are box shifts encoded in the list
+// entry instead of ghost copies: a site image's position is the site position
+// plus the shift, which for atoms is the same floating-point sum the reference
+// forms when it creates a ghost (x + pbc*prd, atom_vec_atomic.cpp:280-345).
+#include "cac_ctx.hpp"
+
+namespace cac {
+
+// ------------------------------------------------------------------ scans
+constexpr int SCAN_BLOCK = 1024;
+
+__global__ void k_scan_block(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n) {
+  __shared__ int s[SCAN_BLOCK];
+  const int t = threadIdx.x;
+  const long i = (long)blockIdx.x * SCAN_BLOCK + t;
+  const int v = (i < n) ? in[i] : 0;
+  s[t] = v;
+  __syncthreads();
+  for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+    int a = (t >= off) ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += a;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[t] - v;
+  if (t == SCAN_BLOCK - 1) bsum[blockIdx.x] = s[t];
+}
+
+__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ boff, int n) {
+  const long i = (long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  if (i < n) out[i] += boff[blockIdx.x];
+}
+
+// Exclusive scan of n ints; *total (host) receives the sum.  Deterministic.
+void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total) {
+  if (n <= 0) {
+    if (total) *total = 0;
+    return;
+  }
+  int nb1 = grid_for(n, SCAN_BLOCK);
+  int nb2 = grid_for(nb1, SCAN_BLOCK);
+  int nb3 = grid_for(nb2, SCAN_BLOCK);
+  if (nb3 > 1) throw HipError{hipErrorInvalidValue, "scan longer than 2^30"};
+  c.scan_tmp.reserve((size_t)2 * nb1 + 2 * nb2 + 4);
+  int* b1 = c.scan_tmp.get();
+  int* o1 = b1 + nb1;
+  int* b2 = o1 + nb1;
+  int* o2 = b2 + nb2;
+  int* b3 = o2 + nb2;
+  k_scan_block<<<nb1, SCAN_BLOCK, 0, c.stream>>>(in, out, b1, n);
+  if (nb1 > 1) {
+    k_scan_block<<<nb2, SCAN_BLOCK, 0, c.stream>>>(b1, o1, b2, nb1);
+    if (nb2 > 1) {
+      k_scan_block<<<1, SCAN_BLOCK, 0, c.stream>>>(b2, o2, b3, nb2);
+      k_scan_add<<<nb2, SCAN_BLOCK, 0, c.stream>>>(o1, o2, nb1);
+    }
+    k_scan_add<<<nb1, SCAN_BLOCK, 0, c.stream>>>(out, o1, n);
+  }
+  if (total) {
+    int last_in = 0, last_out = 0;
+    CAC_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CAC_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CAC_HIP(hipStreamSynchronize(c.stream));
+    *total = last_in + last_out;
+  }
+}
+
+// ------------------------------------------------------------------ pbc
+
+struct Box {
+  double lo[3], hi[3], L[3];
+};
+
+static Box box_of(const Ctx& c) {
+  Box b;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = c.lo[d];
+    b.hi[d] = c.hi[d];
+    b.L[d] = c.L[d];
+  }
+  return b;
+}
+
+// Domain::pbc on owned atoms (domain.cpp:560-640)
+__global__ void k_pbc_atoms(double* __restrict__ x, int A, int P, Box b, ErrorFlags* flags) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A) return;
+  for (int d = 0; d < 3; d++) {
+    double xd = x[(size_t)d * P + p];
+    if (!isfinite(xd)) {
+      atomicOr(&flags->nan, 1);
+      return;
+    }
+    if (xd < b.lo[d]) xd += b.L[d];
+    if (xd >= b.hi[d]) {
+      xd -= b.L[d];
+      xd = fmax(xd, b.lo[d]);
+    }
+    x[(size_t)d * P + p] = xd;
+  }
+}
+
+// Domain::pbc on elements: wrap the centre, shift the nodes rigidly with
+// ElementVecCAC::update_node_coord(i, dim) (domain.cpp:671-737,
+// element_vec_cac.cpp:3772-3784)
+__global__ void k_pbc_elements(double* __restrict__ x, double* __restrict__ ex, int A, int E, int P, Box b,
+                               ErrorFlags* flags) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  for (int d = 0; d < 3; d++) {
+    double* nx = x + (size_t)d * P + A + (size_t)NPE * e;
+    double c = ex[(size_t)d * E + e];
+    bool bad = !isfinite(c);
+    for (int k = 0; k < NPE; k++) bad |= !isfinite(nx[k]);
+    if (bad) {
+      atomicOr(&flags->nan, 1);
+      return;
+    }
+    for (int pass = 0; pass < 2; pass++) {
+      bool moved = false;
+      if (pass == 0 && c < b.lo[d]) {
+        c += b.L[d];
+        moved = true;
+      }
+      if (pass == 1 && c >= b.hi[d]) {
+        c -= b.L[d];
+        c = fmax(c, b.lo[d]);
+        moved = true;
+      }
+      if (moved) {
+        double xt = 0.0;
+        for (int k = 0; k < NPE; k++) xt += nx[k];
+        xt = c - xt / NPE / 1;
+        for (int k = 0; k < NPE; k++) nx[k] += xt;
+      }
+    }
+    ex[(size_t)d * E + e] = c;
+  }
+}
+
+void launch_pbc(Ctx& c) {
+  Box b = box_of(c);
+  if (c.A) k_pbc_atoms<<<grid_for(c.A, 256), 256, 0, c.stream>>>(c.x.get(), c.A, c.P, b, c.d_flags.get());
+  if (c.E)
+    k_pbc_elements<<<grid_for(c.E, 256), 256, 0, c.stream>>>(c.x.get(), c.ex.get(), c.A, c.E, c.P, b,
+                                                              c.d_flags.get());
+}
+
+// Element::update_element_bound_box(check = 1): an element whose bound box grew
+// or shrank more than maxelemchange = 2 in any dimension is fatal.
+__global__ void k_deform(const double* __restrict__ x, double* __restrict__ init, int A, int E, int P, int first,
+                         ErrorFlags* flags) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  for (int d = 0; d < 3; d++) {
+    const double* nx = x + (size_t)d * P + A + (size_t)NPE * e;
+    double mx = -1e30, mn = 1e30;
+    for (int k = 0; k < NPE; k++) {
+      mx = fmax(mx, nx[k]);
+      mn = fmin(mn, nx[k]);
+    }
+    double size = mx - mn;
+    double* ip = init + (size_t)d * E + e;
+    if (first)
+      *ip = size;
+    else if (size / *ip > 2.0 || *ip / size > 2.0)
+      atomicOr(&flags->deform, 1);
+  }
+}
+
+void check_deformation(Ctx& c, bool first) {
+  if (!c.E) return;
+  k_deform<<<grid_for(c.E, 256), 256, 0, c.stream>>>(c.x.get(), c.initsize.get(), c.A, c.E, c.P, first ? 1 : 0,
+                                                      c.d_flags.get());
+}
+
+// ------------------------------------------------------------------ sites and cells
+
+struct Grid {
+  double lo[3], L[3], cinv[3];
+  int nc[3];
+};
+
+static Grid grid_of(const Ctx& c) {
+  Grid g;
+  for (int d = 0; d < 3; d++) {
+    g.lo[d] = c.lo[d];
+    g.L[d] = c.L[d];
+    g.cinv[d] = c.cinv[d];
+    g.nc[d] = c.nc[d];
+  }
+  return g;
+}
+
+// Wrap a position into the box: y = x + s*L with s in {-1,0,1}; returns the
+// cell and the packed shift code (sx+1) + 3(sy+1) + 9(sz+1); -1 on a shift
+// outside [-1, 1] (an element larger than the box).
+__device__ __forceinline__ int wrap_cell(const Grid& g, double* y, int* code) {
+  int cc[3], s[3];
+  for (int d = 0; d < 3; d++) {
+    double f = floor((y[d] - g.lo[d]) / g.L[d]);
+    s[d] = -(int)f;
+    if (s[d] < -1 || s[d] > 1) return -1;
+    y[d] = y[d] + (double)s[d] * g.L[d];
+    int ic = (int)floor((y[d] - g.lo[d]) * g.cinv[d]);
+    cc[d] = ic < 0 ? 0 : (ic >= g.nc[d] ? g.nc[d] - 1 : ic);
+  }
+  *code = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+  return (cc[2] * g.nc[1] + cc[1]) * g.nc[0] + cc[0];
+}
+
+__device__ __forceinline__ void shift_of(int code, const double* L, double* sh) {
+  sh[0] = (double)(code % 3 - 1) * L[0];
+  sh[1] = (double)((code / 3) % 3 - 1) * L[1];
+  sh[2] = (double)(code / 9 - 1) * L[2];
+}
+
+// interpolated position of unit cell u of element e (start from 0, nodes in
+// order, as the reference forms Gauss-point positions, pair_eam.cpp:887-893)
+__device__ __forceinline__ void interp_pos(const double* __restrict__ x, const double* __restrict__ shape, int A,
+                                           int P, int e, int u, double* y) {
+  const double* N = shape + (size_t)u * NPE;
+  const size_t base = (size_t)A + (size_t)NPE * e;
+  for (int d = 0; d < 3; d++) {
+    const double* nx = x + (size_t)d * P + base;
+    double s = 0.0;
+    for (int k = 0; k < NPE; k++) s += N[k] * nx[k];
+    y[d] = s;
+  }
+}
+
+__global__ void k_point_occ(const double* __restrict__ x, int P, Grid g, int* __restrict__ occ,
+                            ErrorFlags* flags) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double y[3] = {x[p], x[(size_t)P + p], x[2 * (size_t)P + p]};
+  int code;
+  int cell = wrap_cell(g, y, &code);
+  if (cell < 0) {
+    atomicOr(&flags->nan, 2);
+    return;
+  }
+  occ[cell] = 1;
+}
+
+__global__ void k_point_sites(const double* __restrict__ x, int P, Grid g, double* __restrict__ tsx,
+                              int* __restrict__ tdesc, int* __restrict__ timg, int* __restrict__ tcell, int S_cap) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double y[3] = {x[p], x[(size_t)P + p], x[2 * (size_t)P + p]};
+  int code;
+  int cell = wrap_cell(g, y, &code);
+  if (cell < 0) {
+    cell = 0;
+    code = 13;
+  }
+  for (int d = 0; d < 3; d++) tsx[(size_t)d * S_cap + p] = y[d];
+  tdesc[p] = p;
+  timg[p] = code;
+  tcell[p] = cell;
+}
+
+// dilate a 0/1 cell mask by +-2 cells along one axis, periodic
+__global__ void k_dilate(const int* __restrict__ in, int* __restrict__ out, Grid g, int axis) {
+  const long cidx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ncell = (long)g.nc[0] * g.nc[1] * g.nc[2];
+  if (cidx >= ncell) return;
+  int ix = (int)(cidx % g.nc[0]);
+  int iy = (int)((cidx / g.nc[0]) % g.nc[1]);
+  int iz = (int)(cidx / ((long)g.nc[0] * g.nc[1]));
+  int v = 0;
+  for (int o = -2; o <= 2; o++) {
+    int c3[3] = {ix, iy, iz};
+    int q = c3[axis] + o;
+    q = (q < 0) ? q + g.nc[axis] : (q >= g.nc[axis] ? q - g.nc[axis] : q);
+    c3[axis] = q;
+    v |= in[((long)c3[2] * g.nc[1] + c3[1]) * g.nc[0] + c3[0]];
+  }
+  out[cidx] = v;
+}
+
+// count / fill the interpolated atoms (non-node unit cells) of each element
+// whose cell lies within two cells of some point.  One workgroup per element;
+// the fill pass places them with a wavefront ballot + workgroup prefix, so the
+// order (element-major, unit-cell order) is deterministic.
+constexpr int VB = 256;
+
+template <bool FILL>
+__global__ __launch_bounds__(VB) void k_virt(const double* __restrict__ x, const double* __restrict__ shape,
+                                             const int* __restrict__ u2g, const int* __restrict__ nearmask, int A,
+                                             int P, int nucell, Grid g, int* __restrict__ ecount,
+                                             const int* __restrict__ eoffset, double* __restrict__ tsx,
+                                             int* __restrict__ tdesc, int* __restrict__ timg,
+                                             int* __restrict__ tcell, int S_cap) {
+  const int e = blockIdx.x;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  __shared__ int wsum[VB / 64];
+  int base = FILL ? (P + eoffset[e]) : 0;
+  int total = 0;
+  for (int u0 = 0; u0 < nucell; u0 += VB) {
+    const int u = u0 + t;
+    int need = 0, cell = 0, code = 13;
+    double y[3] = {0.0, 0.0, 0.0};
+    if (u < nucell && u2g[u] < 0) {
+      interp_pos(x, shape, A, P, e, u, y);
+      cell = wrap_cell(g, y, &code);
+      need = (cell >= 0) && nearmask[cell];
+    }
+    unsigned long long m = __ballot(need);
+    int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wave] = __popcll(m);
+    __syncthreads();
+    int woff = 0, chunk = 0;
+    for (int w = 0; w < VB / 64; w++) {
+      if (w < wave) woff += wsum[w];
+      chunk += wsum[w];
+    }
+    if (FILL && need) {
+      const int s = base + total + woff + rank;
+      for (int d = 0; d < 3; d++) tsx[(size_t)d * S_cap + s] = y[d];
+      tdesc[s] = -(1 + e * nucell + u);
+      timg[s] = code;
+      tcell[s] = cell;
+    }
+    total += chunk;
+    __syncthreads();
+  }
+  if (!FILL && t == 0) ecount[e] = total;
+}
+
+__global__ void k_cell_count(const int* __restrict__ tcell, int S, int* __restrict__ count) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < S) atomicAdd(&count[tcell[s]], 1);
+}
+
+__global__ void k_cell_scatter(const int* __restrict__ tcell, int S, const int* __restrict__ start,
+                               int* __restrict__ cursor, int* __restrict__ perm) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int c = tcell[s];
+  const int slot = atomicAdd(&cursor[c], 1);
+  perm[start[c] + slot] = s;
+}
+
+// the scatter above leaves each cell's sites in arbitrary order: sort every
+// cell's segment by site id so the layout is deterministic
+__global__ void k_cell_sort(const int* __restrict__ start, const int* __restrict__ count, long ncell,
+                            int* __restrict__ perm) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int* a = perm + start[c];
+  const int n = count[c];
+  for (int i = 1; i < n; i++) {
+    int v = a[i], j = i - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      j--;
+    }
+    a[j + 1] = v;
+  }
+}
+
+__global__ void k_apply_perm(const int* __restrict__ perm, int S, int S_cap, const double* __restrict__ tsx,
+                             const int* __restrict__ tdesc, const int* __restrict__ timg, double* __restrict__ sx,
+                             int* __restrict__ sdesc, int* __restrict__ simg, int* __restrict__ point_site,
+                             int* __restrict__ isrow) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S) return;
+  const int s = perm[k];
+  for (int d = 0; d < 3; d++) sx[(size_t)d * S_cap + k] = tsx[(size_t)d * S_cap + s];
+  const int desc = tdesc[s];
+  sdesc[k] = desc;
+  simg[k] = timg[s];
+  isrow[k] = desc >= 0;
+  if (desc >= 0) point_site[desc] = k;
+}
+
+__global__ void k_rows(const int* __restrict__ sdesc, const int* __restrict__ isrow, const int* __restrict__ rowidx,
+                       int S, int* __restrict__ row_point, int* __restrict__ row_site) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S || !isrow[k]) return;
+  const int r = rowidx[k];
+  row_point[r] = sdesc[k];
+  row_site[r] = k;
+}
+
+// One thread per row: scan the 5x5x5 block of cells around the row's cell
+// (cell edge > (rc+skin)/2, periodic), keep every site image within rc + skin
+// except the row's own site, append in scan order (deterministic).
+__global__ __launch_bounds__(256) void k_build_lists(const double* __restrict__ sx, int S_cap,
+                                                     const int* __restrict__ row_site, int R, Grid g,
+                                                     const int* __restrict__ cstart, double cutsq,
+                                                     int* __restrict__ nnum, int* __restrict__ nlist, int cap,
+                                                     ErrorFlags* flags) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int k = row_site[r];
+  const double px = sx[k], py = sx[(size_t)S_cap + k], pz = sx[2 * (size_t)S_cap + k];
+  int ic[3];
+  const double pp[3] = {px, py, pz};
+  for (int d = 0; d < 3; d++) {
+    int v = (int)floor((pp[d] - g.lo[d]) * g.cinv[d]);
+    ic[d] = v < 0 ? 0 : (v >= g.nc[d] ? g.nc[d] - 1 : v);
+  }
+  int n = 0;
+  for (int dz = -2; dz <= 2; dz++) {
+    int cz = ic[2] + dz, tz = 0;
+    if (cz < 0) { cz += g.nc[2]; tz = -1; }
+    else if (cz >= g.nc[2]) { cz -= g.nc[2]; tz = 1; }
+    const double shz = (double)tz * g.L[2];
+    for (int dy = -2; dy <= 2; dy++) {
+      int cy = ic[1] + dy, ty = 0;
+      if (cy < 0) { cy += g.nc[1]; ty = -1; }
+      else if (cy >= g.nc[1]) { cy -= g.nc[1]; ty = 1; }
+      const double shy = (double)ty * g.L[1];
+      for (int dx = -2; dx <= 2; dx++) {
+        int cx = ic[0] + dx, tx = 0;
+        if (cx < 0) { cx += g.nc[0]; tx = -1; }
+        else if (cx >= g.nc[0]) { cx -= g.nc[0]; tx = 1; }
+        const double shx = (double)tx * g.L[0];
+        const int tcode = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+        const long cid = ((long)cz * g.nc[1] + cy) * g.nc[0] + cx;
+        const int j0 = cstart[cid], j1 = cstart[cid + 1];
+        for (int j = j0; j < j1; j++) {
+          const double delx = px - (sx[j] + shx);
+          const double dely = py - (sx[(size_t)S_cap + j] + shy);
+          const double delz = pz - (sx[2 * (size_t)S_cap + j] + shz);
+          const double rsq = delx * delx + dely * dely + delz * delz;
+          if (rsq <= cutsq && !(j == k && tcode == 13)) {
+            if (n < cap) nlist[(size_t)n * R + r] = j | (tcode << IMG_SHIFT);
+            n++;
+          }
+        }
+      }
+    }
+  }
+  nnum[r] = n < cap ? n : cap;
+  atomicMax(&flags->maxcount, n);
+  if (n > cap) atomicOr(&flags->overflow, 1);
+}
+
+static void read_flags(Ctx& c) {
+  CAC_HIP(hipMemcpyAsync(c.h_flags, c.d_flags.get(), sizeof(ErrorFlags), hipMemcpyDeviceToHost, c.stream));
+  CAC_HIP(hipStreamSynchronize(c.stream));
+}
+
+void build_neighbor(Ctx& c) {
+  ScopedTimer tm(c, "neigh_build");
+  c.ago = 0;
+  c.ncalls++;
+  CAC_HIP(hipMemsetAsync(c.d_flags.get(), 0, sizeof(ErrorFlags), c.stream));
+  launch_pbc(c);
+  check_deformation(c, false);
+  // hold positions for check_distance (neighbor.cpp:1732-1776)
+  CAC_HIP(hipMemcpyAsync(c.xhold.get(), c.x.get(), sizeof(double) * 3 * c.P, hipMemcpyDeviceToDevice, c.stream));
+
+  // cell grid: edge > (rc + skin)/2 so a 5x5x5 block covers the cutoff
+  for (int d = 0; d < 3; d++) {
+    int n = (int)floor(c.L[d] / (0.5 * c.cutneigh * (1.0 + 1e-9)));
+    if (n < 5) throw HipError{hipErrorInvalidValue, "box shorter than 2.5 (rc + skin) in some dimension"};
+    c.nc[d] = n;
+    c.cinv[d] = (double)n / c.L[d];
+  }
+  c.ncells = (long)c.nc[0] * c.nc[1] * c.nc[2];
+  if (c.ncells > (1L << 30)) throw HipError{hipErrorInvalidValue, "too many neighbor cells"};
+  Grid g = grid_of(c);
+  const int P = c.P;
+
+  c.occ.reserve(c.ncells);
+  c.nearA.reserve(c.ncells);
+  c.nearB.reserve(c.ncells);
+  CAC_HIP(hipMemsetAsync(c.occ.get(), 0, sizeof(int) * c.ncells, c.stream));
+
+  // pass 1: cells holding a point, dilated by two cells
+  k_point_occ<<<grid_for(P, 256), 256, 0, c.stream>>>(c.x.get(), P, g, c.occ.get(), c.d_flags.get());
+  const int cb = grid_for(c.ncells, 256);
+  k_dilate<<<cb, 256, 0, c.stream>>>(c.occ.get(), c.nearA.get(), g, 0);
+  k_dilate<<<cb, 256, 0, c.stream>>>(c.nearA.get(), c.nearB.get(), g, 1);
+  k_dilate<<<cb, 256, 0, c.stream>>>(c.nearB.get(), c.nearA.get(), g, 2);
+
+  // pass 2: count interpolated atoms near any point, then lay out the sites
+  int nvirt = 0;
+  if (c.E) {
+    c.ecount.reserve(c.E);
+    c.eoffset.reserve(c.E);
+    k_virt<false><<<c.E, VB, 0, c.stream>>>(c.x.get(), c.shape.get(), c.u2g.get(), c.nearA.get(), c.A, P, c.nucell,
+                                           g, c.ecount.get(), nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+    scan_exclusive(c, c.ecount.get(), c.eoffset.get(), c.E, &nvirt);
+  }
+  const int S = P + nvirt;
+  if ((long)S >= (1L << IMG_SHIFT)) throw HipError{hipErrorInvalidValue, "more than 2^27 sites"};
+  const int S_cap = S;
+  c.tsx.reserve((size_t)3 * S_cap);
+  c.tdesc.reserve(S_cap);
+  c.timg.reserve(S_cap);
+  c.tcell.reserve(S_cap);
+  k_point_sites<<<grid_for(P, 256), 256, 0, c.stream>>>(c.x.get(), P, g, c.tsx.get(), c.tdesc.get(), c.timg.get(),
+                                                        c.tcell.get(), S_cap);
+  if (c.E && nvirt)
+    k_virt<true><<<c.E, VB, 0, c.stream>>>(c.x.get(), c.shape.get(), c.u2g.get(), c.nearA.get(), c.A, P, c.nucell, g,
+                                          nullptr, c.eoffset.get(), c.tsx.get(), c.tdesc.get(), c.timg.get(),
+                                          c.tcell.get(), S_cap);
+
+  // counting sort of sites by cell
+  c.cell_count.reserve(c.ncells + 1);
+  c.cell_start.reserve(c.ncells + 1);
+  c.cell_cursor.reserve(c.ncells + 1);
+  CAC_HIP(hipMemsetAsync(c.cell_count.get(), 0, sizeof(int) * (c.ncells + 1), c.stream));
+  CAC_HIP(hipMemsetAsync(c.cell_cursor.get(), 0, sizeof(int) * (c.ncells + 1), c.stream));
+  k_cell_count<<<grid_for(S, 256), 256, 0, c.stream>>>(c.tcell.get(), S, c.cell_count.get());
+  int tot = 0;
+  scan_exclusive(c, c.cell_count.get(), c.cell_start.get(), (int)c.ncells + 1, &tot);
+  c.perm.reserve(S);
+  k_cell_scatter<<<grid_for(S, 256), 256, 0, c.stream>>>(c.tcell.get(), S, c.cell_start.get(), c.cell_cursor.get(),
+                                                         c.perm.get());
+  k_cell_sort<<<grid_for(c.ncells, 256), 256, 0, c.stream>>>(c.cell_start.get(), c.cell_count.get(), c.ncells,
+                                                              c.perm.get());
+  c.sx.reserve((size_t)3 * S_cap);
+  c.sfp.reserve(S_cap);
+  c.sfp32.reserve(S_cap);
+  c.sdesc.reserve(S_cap);
+  c.simg.reserve(S_cap);
+  c.isflag.reserve(S_cap);
+  c.point_site.reserve(P);
+  c.row_point.reserve(P);
+  c.row_site.reserve(P);
+  k_apply_perm<<<grid_for(S, 256), 256, 0, c.stream>>>(c.perm.get(), S, S_cap, c.tsx.get(), c.tdesc.get(),
+                                                       c.timg.get(), c.sx.get(), c.sdesc.get(), c.simg.get(),
+                                                       c.point_site.get(), c.isflag.get());
+  c.rowidx.reserve(S);
+  int nrows = 0;
+  scan_exclusive(c, c.isflag.get(), c.rowidx.get(), S, &nrows);
+  if (nrows != P) throw HipError{hipErrorInvalidValue, "row count mismatch"};
+  k_rows<<<grid_for(S, 256), 256, 0, c.stream>>>(c.sdesc.get(), c.isflag.get(), c.rowidx.get(), S,
+                                                 c.row_point.get(), c.row_site.get());
+  c.S = S;
+  c.nvirt = nvirt;
+
+  // full lists; grow the per-point capacity and rebuild once if a row overflowed
+  if (c.cap == 0) c.cap = 64;
+  c.nnum.reserve(P);
+  for (int attempt = 0; attempt < 3; attempt++) {
+    c.nlist.reserve((size_t)c.cap * P);
+    CAC_HIP(hipMemsetAsync(&c.d_flags.get()->overflow, 0, sizeof(int) * 2, c.stream));
+    k_build_lists<<<grid_for(P, 256), 256, 0, c.stream>>>(c.sx.get(), S_cap, c.row_site.get(), P, g,
+                                                          c.cell_start.get(), c.cutneighsq, c.nnum.get(),
+                                                          c.nlist.get(), c.cap, c.d_flags.get());
+    read_flags(c);
+    if (!c.h_flags->overflow) break;
+    int need = c.h_flags->maxcount;
+    if (need > c.one) throw HipError{hipErrorInvalidValue, "Neighbor list overflow, boost neigh_modify one"};
+    c.cap = ((need + 15) / 16) * 16 + 16;
+  }
+  if (c.h_flags->nan & 1) throw HipError{hipErrorInvalidValue, "Non-numeric positions - simulation unstable"};
+  if (c.h_flags->nan & 2) throw HipError{hipErrorInvalidValue, "an element is larger than the periodic box"};
+  if (c.h_flags->deform) throw HipError{hipErrorInvalidValue, "Simulation unstable! an element deformed by more than 2.0 times"};
+  // entries actually stored
+  {
+    std::vector<int> h(P);
+    CAC_HIP(hipMemcpy(h.data(), c.nnum.get(), sizeof(int) * P, hipMemcpyDeviceToHost));
+    long s = 0;
+    for (int v : h) s += v;
+    c.entries = s;
+  }
+  c.built = true;
+}
+
+// ------------------------------------------------------------------ check_distance (neighbor.cpp:1617-1683)
+
+__global__ void k_check_distance(const double* __restrict__ x, const double* __restrict__ xh, int P, double trig,
+                                 ErrorFlags* flags) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int hit = 0;
+  if (p < P) {
+    double dx = x[p] - xh[p];
+    double dy = x[(size_t)P + p] - xh[(size_t)P + p];
+    double dz = x[2 * (size_t)P + p] - xh[2 * (size_t)P + p];
+    hit = (dx * dx + dy * dy + dz * dz > trig);
+  }
+  if (__ballot(hit) && (threadIdx.x & 63) == 0) atomicOr(&flags->rebuild, 1);
+}
+
+int check_distance(Ctx& c) {
+  CAC_HIP(hipMemsetAsync(&c.d_flags.get()->rebuild, 0, sizeof(int), c.stream));
+  k_check_distance<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.xhold.get(), c.P, c.triggersq,
+                                                             c.d_flags.get());
+  CAC_HIP(hipMemcpyAsync(&c.h_flags->rebuild, &c.d_flags.get()->rebuild, sizeof(int), hipMemcpyDeviceToHost,
+                         c.stream));
+  CAC_HIP(hipStreamSynchronize(c.stream));
+  return c.h_flags->rebuild;
+}
+
+// ------------------------------------------------------------------ export for parity checks
+
+void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
+                  std::vector<int>& ucell, std::vector<int>& img) {
+  const int P = c.P, S = c.S;
+  std::vector<int> nnum(P), rp(P), rs(P), sdesc(S), simg(S);
+  std::vector<int> nl((size_t)c.cap * P);
+  CAC_HIP(hipStreamSynchronize(c.stream));
+  CAC_HIP(hipMemcpy(nnum.data(), c.nnum.get(), sizeof(int) * P, hipMemcpyDeviceToHost));
+  CAC_HIP(hipMemcpy(rp.data(), c.row_point.get(), sizeof(int) * P, hipMemcpyDeviceToHost));
+  CAC_HIP(hipMemcpy(rs.data(), c.row_site.get(), sizeof(int) * P, hipMemcpyDeviceToHost));
+  CAC_HIP(hipMemcpy(sdesc.data(), c.sdesc.get(), sizeof(int) * S, hipMemcpyDeviceToHost));
+  CAC_HIP(hipMemcpy(simg.data(), c.simg.get(), sizeof(int) * S, hipMemcpyDeviceToHost));
+  CAC_HIP(hipMemcpy(nl.data(), c.nlist.get(), sizeof(int) * (size_t)c.cap * P, hipMemcpyDeviceToHost));
+  auto decode = [](int code, int* s) {
+    s[0] = code % 3 - 1;
+    s[1] = (code / 3) % 3 - 1;
+    s[2] = code / 9 - 1;
+  };
+  row.clear(); kind.clear(); owner.clear(); ucell.clear(); img.clear();
+  for (int r = 0; r < P; r++) {
+    const int p = rp[r];
+    int prow;
+    if (p < c.A)
+      prow = p;
+    else {
+      int e = (p - c.A) / NPE, k = (p - c.A) % NPE;
+      prow = c.A + NPE * e + c.n2g[k];
+    }
+    int sp[3];
+    decode(simg[rs[r]], sp);
+    for (int n = 0; n < nnum[r]; n++) {
+      const int ent = nl[(size_t)n * P + r];
+      const int j = ent & SITE_MASK, t = ent >> IMG_SHIFT;
+      int ts[3], ss[3];
+      decode(t, ts);
+      decode(simg[j], ss);
+      const int d = sdesc[j];
+      int kd, ow, uc;
+      if (d >= 0 && d < c.A) {
+        kd = 0; ow = d; uc = 0;
+      } else if (d >= 0) {
+        int e = (d - c.A) / NPE, k = (d - c.A) % NPE;
+        kd = 1; ow = e; uc = c.g2u[c.n2g[k]];
+      } else {
+        int q = -d - 1;
+        kd = 1; ow = q / c.nucell; uc = q % c.nucell;
+      }
+      row.push_back(prow);
+      kind.push_back(kd);
+      owner.push_back(ow);
+      ucell.push_back(uc);
+      for (int dd = 0; dd < 3; dd++) img.push_back(ss[dd] + ts[dd] - sp[dd]);
+    }
+  }
+}
+
+}  // namespace cac

@@ -1,0 +1,129 @@
+/*
+ * cac_hip.h -- C ABI of the MI355X CAC force + neighbor hot path (libcachip.so).
+ *
+ * Plain pointers and sizes only.  One context (cac_hip_ctx) holds one CAC model
+ * resident in HBM: atoms, elements, neighbor lists and potential tables, plus
+ * the HIP stream every call is ordered on.  Host arrays use the reference's
+ * layouts: atom x/v/f are [n][3] (atom.h:36-39); element nodex/nodev/nodef/gaussf
+ * are [n][apc=1][npe=8][3] (element.h:47-95, element_vec_cac.cpp:175-180).
+ * Gauss-cell forces are returned per Gauss cell g in the reference's igcell order.
+ *
+ * Every function returns 0 on success or a CAC_HIP_E* code; the message is in
+ * cac_hip_last_error().  Calls on one context are host-thread affine.
+ */
+#ifndef CAC_HIP_H
+#define CAC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cac_hip_ctx cac_hip_ctx;
+
+enum {
+  CAC_HIP_OK = 0,
+  CAC_HIP_EARG = 1,      /* bad argument (maps to error->all) */
+  CAC_HIP_EOVERFLOW = 2, /* a neighbor row exceeds the per-point capacity ("neigh_modify one") */
+  CAC_HIP_ENAN = 3,      /* non-numeric coordinates (domain.cpp:541-546, nbin.cpp:130-131) */
+  CAC_HIP_EDEFORM = 4,   /* element deformed more than 2x (element.cpp:1452-1461) */
+  CAC_HIP_EOOM = 5,      /* device allocation failed */
+  CAC_HIP_EHIP = 6,      /* HIP runtime error */
+  CAC_HIP_ESTATE = 7,    /* call out of order (e.g. compute before a neighbor build) */
+  CAC_HIP_ENODEV = 8     /* no HIP device */
+};
+
+enum { CAC_HIP_UNITS_LJ = 0, CAC_HIP_UNITS_METAL = 1 };           /* update.cpp:112-184 */
+enum { CAC_HIP_FP64 = 0, CAC_HIP_MIXED = 1 };                      /* arithmetic of the pair terms */
+enum { CAC_HIP_NODEF_CONSISTENT = 0, CAC_HIP_NODEF_LUMPED = 1, CAC_HIP_NODEF_NODEONLY = 2 }; /* element.cpp:1031-1033 */
+
+/* ---- context: replaces the CAC/Atom/Element/Force singletons of one rank (cac.cpp:36-250) ---- */
+int cac_hip_create(cac_hip_ctx **ctx, int device, int units);
+int cac_hip_destroy(cac_hip_ctx *ctx);
+const char *cac_hip_last_error(const cac_hip_ctx *ctx);
+
+/* ---- model: replaces read_data / create_box / mass / element type setup ----
+ * box: Domain boxlo/boxhi, orthogonal and periodic in x, y, z (domain.cpp:526-738)
+ * masses: atom->mass[1..ntypes] (atom.h)
+ * element type: ElementVecCAC::set_element_types "itype Hex 1 ncx ncy ncz ngx ngy ngz"
+ *   (element_vec_cac.cpp:1446-1741); this build takes Hex, apc 1, 2x2x2 Gauss cells. */
+int cac_hip_set_box(cac_hip_ctx *ctx, const double *lo, const double *hi);
+int cac_hip_set_masses(cac_hip_ctx *ctx, int ntypes, const double *mass);
+int cac_hip_add_etype(cac_hip_ctx *ctx, int itype, int ncx, int ncy, int ncz, int ngx, int ngy, int ngz);
+/* atom->x, v [n][3] and type [n] (AtomVecAtomic, atom_vec_atomic.cpp:26); v may be NULL */
+int cac_hip_upload_atoms(cac_hip_ctx *ctx, int n, const double *x, const double *v, const int *type);
+/* element->etype [n], ctype [n][1], nodex/nodev [n][1][8][3] (ElementVecCAC, element_vec_cac.cpp:300-391) */
+int cac_hip_upload_elements(cac_hip_ctx *ctx, int n, const int *etype, const int *ctype, const double *nodex,
+                            const double *nodev);
+
+/* ---- pair styles: replace Pair::settings/coeff/init_one/init_style ----
+ * lj/cut: PairLJCut::init_one (pair_lj_cut.cpp:142-166), single type pair, no shift.
+ * eam/alloy: PairEAMAlloy::read_file + file2array + PairEAM::array2spline
+ *   (pair_eam_alloy.cpp:100-315, pair_eam.cpp:561-614); tables are the setfl
+ *   arrays F(rho)[nrho], f(r)[nr], r*phi(r)[nr] of one element. */
+int cac_hip_pair_lj(cac_hip_ctx *ctx, double epsilon, double sigma, double cut);
+int cac_hip_pair_eam(cac_hip_ctx *ctx, int nrho, double drho, int nr, double dr, double cut, const double *frho,
+                     const double *rhor, const double *z2r);
+/* reads a one-element setfl file the way PairEAMAlloy::read_file does */
+int cac_hip_pair_eam_setfl(cac_hip_ctx *ctx, const char *path);
+int cac_hip_set_precision(cac_hip_ctx *ctx, int mode);
+int cac_hip_set_nodal_force_style(cac_hip_ctx *ctx, int style);
+
+/* ---- neighbor: replaces the neighbor request (NeighRequest half/bin/newton,
+ * neigh_request.cpp:18-57) and Neighbor::init/decide/build (neighbor.cpp:203-290,
+ * 1584-1873; NBinStandard, NStencilHalfBin3d, NPairHalfBinNewton).
+ * skin/every/delay/check/one as "neighbor" and "neigh_modify" (neighbor.cpp:61-69). */
+int cac_hip_neighbor_settings(cac_hip_ctx *ctx, double skin, int every, int delay, int check, int one);
+/* Domain::pbc + comm setup_borders/borders + Neighbor::build, all on device */
+int cac_hip_neigh_build(cac_hip_ctx *ctx);
+/* Neighbor::decide (+ check_distance): *flag = 1 when a rebuild is due */
+int cac_hip_neigh_decide(cac_hip_ctx *ctx, int *flag);
+/* size of the built lists: rows (atoms + Gauss points), stored entries, sites, capacity */
+int cac_hip_neigh_info(cac_hip_ctx *ctx, int *rows, long *entries, int *sites, int *capacity);
+/* Export the full per-point lists for parity checks.  Rows are points: atom a is
+ * row a, Gauss cell g of element e is row natoms + 8e + g.  Entry k of the export
+ * is (row, site kind 0 atom / 1 virtual atom, owner index, unit cell, image ix iy iz)
+ * with image = integer box shifts applied to the owner's position.  Pass NULL to
+ * query the count in *n. */
+int cac_hip_neigh_export(cac_hip_ctx *ctx, long *n, int *row, int *kind, int *owner, int *ucell, int *img);
+
+/* ---- force: replaces Verlet::force_clear + Pair::compute(eflag, vflag) + the
+ * reverse comm + Element::compute_nodef (verlet.cpp:192-212, 322-344;
+ * pair_eam.cpp:618-1032, pair_lj_cut.cpp:227-417, element.cpp:2178-2246).
+ * eflag: ENERGY_GLOBAL = 1, ENERGY_ATOM = 2; vflag: VIRIAL_PAIR = 1, VIRIAL_FDOTR = 2 (force.h:11-19) */
+int cac_hip_compute(cac_hip_ctx *ctx, int eflag, int vflag);
+/* Pair::eng_vdwl and Pair::virial[6] of the last compute with eflag / vflag */
+int cac_hip_get_energy(cac_hip_ctx *ctx, double *eng_vdwl, double *virial);
+
+/* ---- integrate: replaces FixNVE::initial_integrate / final_integrate (fix_nve.cpp:62-217) ---- */
+int cac_hip_nve_initial(cac_hip_ctx *ctx, double dt);
+int cac_hip_nve_final(cac_hip_ctx *ctx, double dt);
+
+/* ---- the whole loop: Verlet::setup + Verlet::run (verlet.cpp:65-360) on device.
+ * thermo rows (step, pe, ke, etotal, press) every thermo_every steps and on the
+ * last step; thermo may be NULL. */
+int cac_hip_setup(cac_hip_ctx *ctx, int eflag, int vflag);
+int cac_hip_run(cac_hip_ctx *ctx, int nsteps, double dt, int thermo_every, double *thermo);
+
+/* ---- observables: ComputePE / ComputeTemp / ComputePressure scalars (compute_pe.cpp:46-66,
+ * compute_temp.cpp:73-157, compute_pressure.cpp:140-180) */
+int cac_hip_thermo(cac_hip_ctx *ctx, double *pe, double *ke, double *temp, double *press);
+
+/* ---- host <-> device ---- */
+int cac_hip_download_atoms(cac_hip_ctx *ctx, double *x, double *v, double *f);
+int cac_hip_download_elements(cac_hip_ctx *ctx, double *nodex, double *nodev, double *nodef, double *gaussf);
+int cac_hip_download_rho(cac_hip_ctx *ctx, double *atomrho, double *noderho);
+int cac_hip_counts(cac_hip_ctx *ctx, int *natoms, int *nelements, int *npoints);
+int cac_hip_sync(cac_hip_ctx *ctx);
+/* Timings of the last cac_hip_run: average microseconds per launch of the named
+ * kernel group ("eam_density", "eam_embed", "eam_force", "lj_force", "site_refresh",
+ * "nodef", "nve", "neigh_build"), and the launch count; measured with hipEvents
+ * on the context's stream when timing is enabled. */
+int cac_hip_enable_timing(cac_hip_ctx *ctx, int on);
+int cac_hip_kernel_time(cac_hip_ctx *ctx, const char *name, double *avg_us, long *launches);
+/* neighbor list build / rebuild statistics: builds, dangerous builds (neighbor.cpp:1681) */
+int cac_hip_neigh_stats(cac_hip_ctx *ctx, int *ncalls, int *ndanger);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

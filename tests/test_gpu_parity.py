@@ -1,0 +1,189 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on the same inputs.
+
+Bars: neighbor lists equal as sorted per-point sets (exact); FP64 forces,
+energies, virials and densities within 1e-12 relative (max |a - b| over the
+array divided by max |b|); mixed precision forces within 1e-5 of FP64.
+"""
+
+import numpy as np
+import pytest
+
+from cac_amd import decks
+from tests import brute
+
+pytestmark = pytest.mark.gpu
+
+TAB = decks.cu_eam_tables()
+G2U9 = None
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def oracle_model(deck, pair, skin, **kw):
+    from oracle import oracle as O
+    o = O.Oracle(deck, pair, skin=skin, eam_tables=TAB if pair == "eam" else None, **kw)
+    return o
+
+
+def gpu_model(deck, pair, skin, **kw):
+    import cac_amd
+    return cac_amd.Model(deck, pair, skin=skin, eam_tables=TAB if pair == "eam" else None, **kw)
+
+
+def oracle_sets(o, deck):
+    from oracle import oracle as O
+    p, k, ow, u, img = o.half_list()
+    return O.canonical_full_sets(deck, p, k, ow, u, img, brute.corner_ucells(deck.ncell))
+
+
+def check_forces(o, g, deck, tol=1e-12):
+    _, _, fo = o.atoms()
+    _, _, fg = g.atoms()
+    if deck.natoms:
+        assert rel(fg, fo) < tol, rel(fg, fo)
+    if deck.nelements:
+        eo, eg = o.elements(), g.elements()
+        assert rel(eg["gaussf"], eo["gaussf"]) < tol, rel(eg["gaussf"], eo["gaussf"])
+        assert rel(eg["nodef"], eo["nodef"]) < tol, rel(eg["nodef"], eo["nodef"])
+    e_o, v_o = o.energy()
+    e_g, v_g = g.energy()
+    assert abs(e_g - e_o) <= tol * abs(e_o), (e_g, e_o)
+    assert rel(v_g, v_o) < tol, (v_g, v_o)
+
+
+def setup_pair(deck, pair, skin, **kw):
+    o = oracle_model(deck, pair, skin, **kw)
+    o.setup()
+    g = gpu_model(deck, pair, skin, **kw)
+    g.setup()
+    return o, g
+
+
+# ------------------------------------------------------------------ single force evaluations
+
+def test_lj_atoms_sets_and_forces():
+    deck = decks.lj_block(n=6, jitter=0.05, T=0.1, seed=3)
+    o, g = setup_pair(deck, "lj", 0.3, delay=0)
+    assert np.array_equal(g.full_sets(), oracle_sets(o, deck))
+    check_forces(o, g, deck)
+
+
+def test_eam_atoms_sets_forces_density():
+    a = 3.615
+    rng = np.random.default_rng(5)
+    x = np.mod(decks.fcc_sites(a, (6, 6, 6)) + rng.normal(0, 0.05, (864, 3)), 6 * a)
+    deck = decks.Deck(units="metal", boxlo=np.zeros(3), boxhi=np.full(3, 6 * a), masses=np.array([63.546]), x=x,
+                      v=np.zeros_like(x), atype=np.ones(864, dtype=np.int32))
+    o, g = setup_pair(deck, "eam", 2.0)
+    assert np.array_equal(g.full_sets(), oracle_sets(o, deck))
+    check_forces(o, g, deck)
+    assert rel(g.rho()[0], o.rho()[0]) < 1e-13
+
+
+@pytest.mark.parametrize("nc,box,core,skin", [(3, (3, 3, 3), 8, 1.0), (5, (2, 2, 2), 4, 2.0), (9, (2, 2, 2), 4, 2.0)])
+def test_cac_eam_sets_forces(nc, box, core, skin):
+    deck = decks.cac_eam(box_cells=box, core_cells=core, ncell=nc, jitter=0.03, T=300.0, seed=nc + 1)
+    o, g = setup_pair(deck, "eam", skin)
+    assert np.array_equal(g.full_sets(), oracle_sets(o, deck))
+    check_forces(o, g, deck)
+    ar_o, nr_o = o.rho()
+    ar_g, nr_g = g.rho()
+    assert rel(ar_g, ar_o) < 1e-13 and rel(nr_g, nr_o) < 1e-13
+
+
+def test_cac_lj_sets_forces():
+    rng = np.random.default_rng(1)
+    L, x, cells, nodes = decks.cac_geometry(1.55, 3, (4, 4, 4), 20)
+    nodes = nodes + rng.normal(0, 0.005, nodes.shape)
+    x = np.mod(x + rng.normal(0, 0.005, x.shape), L)
+    E = nodes.shape[0]
+    deck = decks.Deck(units="lj", boxlo=np.zeros(3), boxhi=L, masses=np.array([1.0]), x=x, v=np.zeros_like(x),
+                      atype=np.ones(len(x), dtype=np.int32), ncell=(3, 3, 3), nodex=nodes, nodev=np.zeros_like(nodes),
+                      etype=np.ones(E, dtype=np.int32), ctype=np.ones(E, dtype=np.int32))
+    o, g = setup_pair(deck, "lj", 0.3, delay=0)
+    assert np.array_equal(g.full_sets(), oracle_sets(o, deck))
+    check_forces(o, g, deck)
+
+
+@pytest.mark.parametrize("style", [1, 2])
+def test_nodal_force_styles(style):
+    deck = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=300.0, seed=9)
+    o, g = setup_pair(deck, "eam", 1.0, nodal_force_style=style)
+    check_forces(o, g, deck)
+
+
+# ------------------------------------------------------------------ full-size workloads
+
+def test_cac_eam_full_workload():
+    """The 32,076-atom + 1,748-element EAM workload: sets and forces."""
+    deck = decks.cac_eam()
+    o, g = setup_pair(deck, "eam", 2.0)
+    assert np.array_equal(g.full_sets(), oracle_sets(o, deck))
+    check_forces(o, g, deck)
+
+
+def test_lj_full_workload_momentum():
+    """32,000-atom LJ: forces equal the oracle's and sum to zero."""
+    deck = decks.lj_block()
+    o, g = setup_pair(deck, "lj", 0.3, delay=0)
+    check_forces(o, g, deck)
+    _, _, fg = g.atoms()
+    assert np.abs(fg.sum(axis=0)).max() < 1e-9
+
+
+# ------------------------------------------------------------------ dynamics
+
+def test_cac_trajectory_matches_oracle():
+    deck = decks.cac_eam(box_cells=(2, 2, 2), core_cells=4, ncell=9, jitter=0.02, T=300.0, seed=4)
+    o, g = setup_pair(deck, "eam", 2.0, delay=0)
+    tho = o.run(20, 0.001, thermo_every=10)
+    thg = g.run(20, 0.001, thermo_every=10)
+    assert np.allclose(thg[:, 1:4], tho[:, 1:4], rtol=1e-10, atol=1e-9)
+    xo, vo, _ = o.atoms()
+    xg, vg, _ = g.atoms()
+    assert np.abs(xg - xo).max() < 1e-10
+    eo, eg = o.elements(), g.elements()
+    assert np.abs(eg["nodex"] - eo["nodex"]).max() < 1e-10
+
+
+def test_lj_rebuild_schedule_matches_oracle():
+    deck = decks.lj_block(n=6, jitter=0.02, T=0.5, seed=8)
+    o, g = setup_pair(deck, "lj", 0.3, delay=0)
+    o.run(60, 0.005)
+    g.run(60, 0.005)
+    assert g.neigh_stats() == o.neigh_stats()
+    xo, _, _ = o.atoms()
+    xg, _, _ = g.atoms()
+    assert np.abs(xg - xo).max() < 1e-9
+
+
+def test_run_is_bitwise_deterministic():
+    deck = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=300.0, seed=2)
+    out = []
+    for _ in range(2):
+        g = gpu_model(deck, "eam", 2.0)
+        g.setup()
+        g.run(15, 0.001)
+        out.append((g.atoms()[0], g.elements()["nodex"], g.energy()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+# ------------------------------------------------------------------ mixed precision
+
+def test_mixed_precision_force_error():
+    import cac_amd
+    deck = decks.cac_eam(box_cells=(2, 2, 2), core_cells=4, ncell=9, jitter=0.03, T=300.0, seed=6)
+    g64 = gpu_model(deck, "eam", 2.0)
+    g64.setup()
+    g32 = gpu_model(deck, "eam", 2.0, precision=cac_amd.MIXED)
+    g32.setup()
+    f64 = np.concatenate([g64.atoms()[2], g64.elements()["gaussf"].reshape(-1, 3)])
+    f32 = np.concatenate([g32.atoms()[2], g32.elements()["gaussf"].reshape(-1, 3)])
+    assert rel(f32, f64) < 1e-5
+    e64, _ = g64.energy()
+    e32, _ = g32.energy()
+    assert abs(e32 - e64) < 1e-6 * abs(e64)
