@@ -26,7 +26,7 @@ namespace cac {
 
 constexpr int NPE = 8;
 constexpr int IMG_SHIFT = 27;
-constexpr int SITE_MASK = (1 << IMG_SHIFT) - 1;
+constexpr unsigned SITE_MASK = (1u << IMG_SHIFT) - 1u;  // list entry = site | image << 27, read as unsigned
 constexpr int MAXT = 8;
 
 struct HipError {

@@ -441,7 +441,7 @@ __global__ __launch_bounds__(256) void k_build_lists(const double* __restrict__ 
           const double delz = pz - (sx[2 * (size_t)S_cap + j] + shz);
           const double rsq = delx * delx + dely * dely + delz * delz;
           if (rsq <= cutsq && !(j == k && tcode == 13)) {
-            if (n < cap) nlist[(size_t)n * R + r] = j | (tcode << IMG_SHIFT);
+            if (n < cap) nlist[(size_t)n * R + r] = (int)((unsigned)j | ((unsigned)tcode << IMG_SHIFT));
             n++;
           }
         }
@@ -636,8 +636,8 @@ void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::ve
     int sp[3];
     decode(simg[rs[r]], sp);
     for (int n = 0; n < nnum[r]; n++) {
-      const int ent = nl[(size_t)n * P + r];
-      const int j = ent & SITE_MASK, t = ent >> IMG_SHIFT;
+      const unsigned ent = (unsigned)nl[(size_t)n * P + r];
+      const int j = (int)(ent & SITE_MASK), t = (int)(ent >> IMG_SHIFT);
       int ts[3], ss[3];
       decode(t, ts);
       decode(simg[j], ss);

@@ -113,10 +113,10 @@ __global__ __launch_bounds__(256) void k_eam_density(PairArgs a, const double* _
   const int n = a.nnum[r];
   double rho = 0.0;
   for (int jj = 0; jj < n; jj++) {
-    const int ent = a.nlist[(size_t)jj * a.R + r];
-    const int j = ent & SITE_MASK;
+    const unsigned ent = (unsigned)a.nlist[(size_t)jj * a.R + r];
+    const int j = (int)(ent & SITE_MASK);
     double shx, shy, shz;
-    img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
+    img_shift((int)(ent >> IMG_SHIFT), a.L, shx, shy, shz);
     const double delx = px - (a.sx[j] + shx);
     const double dely = py - (a.sx[(size_t)a.S_cap + j] + shy);
     const double delz = pz - (a.sx[2 * (size_t)a.S_cap + j] + shz);
@@ -205,10 +205,10 @@ __global__ __launch_bounds__(256) void k_eam_force(PairArgs a, EamForceArgs q, d
   double fx = 0.0, fy = 0.0, fz = 0.0;
   double ephi = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
   for (int jj = 0; jj < n; jj++) {
-    const int ent = a.nlist[(size_t)jj * a.R + r];
-    const int j = ent & SITE_MASK;
+    const unsigned ent = (unsigned)a.nlist[(size_t)jj * a.R + r];
+    const int j = (int)(ent & SITE_MASK);
     double shx, shy, shz;
-    img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
+    img_shift((int)(ent >> IMG_SHIFT), a.L, shx, shy, shz);
     const double delx = px - (a.sx[j] + shx);
     const double dely = py - (a.sx[(size_t)a.S_cap + j] + shy);
     const double delz = pz - (a.sx[2 * (size_t)a.S_cap + j] + shz);
@@ -324,10 +324,10 @@ __global__ __launch_bounds__(256) void k_lj_force(PairArgs a, LjArgs q, double* 
   double ev = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
   const float l1 = (float)q.lj1, l2 = (float)q.lj2, l3 = (float)q.lj3, l4 = (float)q.lj4;
   for (int jj = 0; jj < n; jj++) {
-    const int ent = a.nlist[(size_t)jj * a.R + r];
-    const int j = ent & SITE_MASK;
+    const unsigned ent = (unsigned)a.nlist[(size_t)jj * a.R + r];
+    const int j = (int)(ent & SITE_MASK);
     double shx, shy, shz;
-    img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
+    img_shift((int)(ent >> IMG_SHIFT), a.L, shx, shy, shz);
     const double delx = px - (a.sx[j] + shx);
     const double dely = py - (a.sx[(size_t)a.S_cap + j] + shy);
     const double delz = pz - (a.sx[2 * (size_t)a.S_cap + j] + shz);
