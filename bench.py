@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--inner-skin", type=float, default=-1.0,
+                    help="prune margin delta of the inner list (A); < 0 takes 0.25 * skin")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -93,6 +95,9 @@ def main():
     f64 = force_vector(m64)
     e64, _ = m64.energy()
     mmx = cac_amd.Model(deck, "eam", skin=skin, eam_tables=tab, precision=cac_amd.MIXED)
+    if args.inner_skin >= 0:
+        m64.set_inner_skin(args.inner_skin)
+        mmx.set_inner_skin(args.inner_skin)
     mmx.setup()
     fmx = force_vector(mmx)
     emx, _ = mmx.energy()
@@ -102,26 +107,34 @@ def main():
     rms_rel_err = float(np.sqrt(np.mean(df ** 2)) / np.sqrt(np.mean(fmag ** 2)))
     info = mmx.neigh_info()
     nbar = info["entries"] / info["rows"]
+    _, nbar_in, delta = mmx.inner_stats()
 
+    b0, ib0 = mmx.neigh_stats()[0], mmx.inner_stats()[0]
     t_mx = timed_run(mmx, args.steps, args.warmup, dt)
     t_64 = timed_run(m64, args.steps, args.warmup, dt)
+    nsteps_mx = args.steps + args.warmup
+    rebuilds = (mmx.neigh_stats()[0] - b0) / nsteps_mx
+    prunes = (mmx.inner_stats()[0] - ib0) / nsteps_mx
     value = P * args.steps / t_mx
     value64 = P * args.steps / t_64
 
     # per-kernel durations from hipEvents on the context stream (separate short run)
     mmx.enable_timing(True)
     mmx.run(args.profile_steps, dt, thermo=False)
-    kt = {k: mmx.kernel_time(k) for k in ("eam_density", "eam_embed", "eam_force", "site_refresh", "nodef", "nve")}
+    names = ("eam_density", "eam_embed", "eam_force", "site_refresh", "nodef", "nve", "neigh_build", "inner_build")
+    kt = {k: mmx.kernel_time(k) for k in names}
     mmx.enable_timing(False)
     m64.enable_timing(True)
     m64.run(args.profile_steps, dt, thermo=False)
     kt64 = {k: m64.kernel_time(k) for k in ("eam_density", "eam_embed", "eam_force")}
     m64.enable_timing(False)
 
-    # dominant kernel: EAM force pass.  Algorithmic bytes per row: own position
-    # (24) + own F' (8) + force write (24) + 4 bytes per list entry
+    # dominant kernel: EAM force pass (mixed).  Algorithmic bytes per launch:
+    # per row its site index, point index and inner-list length (12) and the
+    # FP64 force write (24); 4 bytes per inner-list entry; each site record
+    # (x, y, z, F' as FP32, 16 bytes) read once.
     force_us = kt["eam_force"][0]
-    bytes_force = P * (56.0 + 4.0 * nbar)
+    bytes_force = P * (36.0 + 4.0 * nbar_in) + 16.0 * info["sites"]
     achieved = bytes_force / (force_us * 1e-6) / 1e9
     peak = 8000.0
 
@@ -147,12 +160,14 @@ def main():
         "data": "synthetic (seeded FCC Cu CAC deck, analytic EAM tabulated as setfl)",
         "config": {"workload": args.workload, "description": desc, "atoms": deck.natoms,
                    "elements": deck.nelements, "atom_equivalents": P, "represented_sites": deck.represented_sites,
-                   "skin": skin, "dt_ps": dt, "neighbor_mean_full": round(nbar, 2), "sites": info["sites"]},
+                   "skin": skin, "dt_ps": dt, "neighbor_mean_full": round(nbar, 2), "sites": info["sites"],
+                   "inner_delta": delta, "inner_mean": round(nbar_in, 2),
+                   "outer_builds_per_step": round(rebuilds, 4), "inner_prunes_per_step": round(prunes, 4)},
         "fp64": {"value": value64, "ms_per_step": 1000.0 * t_64 / args.steps,
                  "speedup_mixed_over_fp64": value / value64},
         "force_error_vs_fp64": {"max_rel": max_rel_err, "rms_rel": rms_rel_err,
                                 "energy_rel": abs(emx - e64) / abs(e64)},
-        "kernels_us": {k: round(v[0], 3) for k, v in kt.items()},
+        "kernels_us": {k: round(v[0], 3) for k, v in kt.items() if v[1]},
         "kernels_us_fp64": {k: round(v[0], 3) for k, v in kt64.items()},
         "roofline": {"bound": "hbm", "kernel": "eam_force (mixed)", "achieved": achieved, "peak": peak,
                      "unit": "GB/s", "frac": achieved / peak, "traffic": None,

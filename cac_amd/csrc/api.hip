@@ -129,6 +129,7 @@ void neighbor_init(Ctx& c) {  // neighbor.cpp:241-283 with one cutoff
   c.cutneigh = c.cutforce + (c.cutforce > 0.0 ? c.skin : 0.0);
   c.cutneighsq = c.cutneigh * c.cutneigh;
   c.triggersq = 0.25 * c.skin * c.skin;
+  c.inner_skin = c.inner_skin_set < 0.0 ? 0.25 * c.skin : std::min(c.inner_skin_set, c.skin);
 }
 
 int require_model(Ctx& c) {
@@ -148,6 +149,14 @@ int require_model(Ctx& c) {
 }
 
 void compute_forces(Ctx& c, int eflag, int vflag) {
+  // the forward comm of a step without rebuild: refresh the sites, and re-prune
+  // the inner list if a point has moved half its buffer
+  if (c.x_dirty) {
+    site_refresh(c);
+    int inner = 0;
+    check_distance(c, false, &inner);
+    if (inner) build_inner(c);
+  }
   // force_clear is implicit: every row writes its own force and every element
   // its own nodal forces
   pair_compute(c, eflag, vflag);
@@ -180,6 +189,8 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     c.d_flags.reserve(1);
     CAC_HIP(hipMemsetAsync(c.d_flags.get(), 0, sizeof(ErrorFlags), c.stream));
     c.units = units;
+    if (const char* e = std::getenv("CAC_HIP_TPR")) c.tpr = std::atoi(e);
+    if (const char* e = std::getenv("CAC_HIP_GRID")) c.grid_cap = std::max(8, std::atoi(e));
     if (units == CAC_HIP_UNITS_LJ) {
       c.boltz = 1.0;
       c.mvv2e = 1.0;
@@ -346,6 +357,7 @@ void finalize_points(Ctx& c) {
   upload(c.ptype, ht, c.stream);
   c.f.reserve((size_t)3 * P);
   c.xhold.reserve((size_t)3 * P);
+  c.xhold_in.reserve((size_t)3 * P);
   c.rho.reserve(P);
   CAC_HIP(hipMemsetAsync(c.f.get(), 0, sizeof(double) * 3 * P, c.stream));
   if (E) {
@@ -439,17 +451,24 @@ int cac_hip_pair_eam(cac_hip_ctx* h, int nrho, double drho, int nr, double dr, d
     upload(c.frho_spl, fs, c.stream);
     upload(c.rhor_spl, rs, c.stream);
     upload(c.z2r_spl, zs, c.stream);
-    // FP32 copies for the mixed path: density needs rhor coefficients 3..6;
-    // force needs rhor' (0..2), z2r' (0..2) and z2r (3..6) of each knot
-    std::vector<float> r32((size_t)(nr + 1) * 4), f32((size_t)(nr + 1) * 10);
-    for (int m = 0; m <= nr; m++) {
-      for (int q = 0; q < 4; q++) r32[(size_t)m * 4 + q] = (float)rs[(size_t)m * 7 + 3 + q];
-      for (int q = 0; q < 3; q++) f32[(size_t)m * 10 + q] = (float)rs[(size_t)m * 7 + q];
-      for (int q = 0; q < 3; q++) f32[(size_t)m * 10 + 3 + q] = (float)zs[(size_t)m * 7 + q];
-      for (int q = 0; q < 4; q++) f32[(size_t)m * 10 + 6 + q] = (float)zs[(size_t)m * 7 + 3 + q];
-    }
-    upload(c.rhor_spl32, r32, c.stream);
-    upload(c.z2r_spl32, f32, c.stream);
+    // FP32 table of the mixed path, staged in LDS by the pair kernels: per knot
+    // the value coefficients c3..c6 of rhor, then of z2r (derivatives are
+    // formed from them)
+    std::vector<float> t32((size_t)(nr + 1) * 8);
+    for (int m = 0; m <= nr; m++)
+      for (int q = 0; q < 4; q++) {
+        t32[(size_t)m * 8 + q] = (float)rs[(size_t)m * 7 + 3 + q];
+        t32[(size_t)m * 8 + 4 + q] = (float)zs[(size_t)m * 7 + 3 + q];
+      }
+    std::vector<double> t64((size_t)(nr + 1) * 8);
+    for (int m = 0; m <= nr; m++)
+      for (int q = 0; q < 4; q++) {
+        t64[(size_t)m * 8 + q] = rs[(size_t)m * 7 + 3 + q];
+        t64[(size_t)m * 8 + 4 + q] = zs[(size_t)m * 7 + 3 + q];
+      }
+    if ((size_t)(nr + 1) * 64 > 160 * 1024) return fail(h, CAC_HIP_EARG, "nr too large for the LDS spline table");
+    upload(c.tab32, t32, c.stream);
+    upload(c.tab64, t64, c.stream);
     c.built = false;
     return CAC_HIP_OK;
   });
@@ -524,16 +543,16 @@ int cac_hip_neigh_build(cac_hip_ctx* h) {
 int cac_hip_neigh_decide(cac_hip_ctx* h, int* flag) {
   return guarded(h, [&](Ctx& c) -> int {
     if (!c.built) return fail(h, CAC_HIP_ESTATE, "neighbor list not built");
-    int f = 0;
+    int f = 0, inner = 0;
     c.ago++;
-    if (c.ago >= c.delay && c.ago % c.every == 0) {
-      if (!c.check)
-        f = 1;
-      else {
-        f = check_distance(c);
-        if (f && c.ago == std::max(c.every, c.delay)) c.ndanger++;
-      }
+    const bool due = c.ago >= c.delay && c.ago % c.every == 0;
+    if (due && !c.check)
+      f = 1;
+    else {
+      f = check_distance(c, due && c.check, &inner);
+      if (f && c.ago == std::max(c.every, c.delay)) c.ndanger++;
     }
+    c.inner_due = inner;
     if (flag) *flag = f;
     return CAC_HIP_OK;
   });
@@ -659,20 +678,23 @@ int cac_hip_run(cac_hip_ctx* h, int nsteps, double dt, int thermo_every, double*
       const bool last = s == nsteps - 1;
       const bool evstep = th && ((thermo_every > 0 && c.ntimestep % thermo_every == 0) || last);
       nve_initial(c, dt);
-      int flag = 0;
+      // Neighbor::decide: the reference's trigger when due, and every step the
+      // inner list's own displacement test
+      int flag = 0, inner = 0;
       c.ago++;
-      if (c.ago >= c.delay && c.ago % c.every == 0) {
-        if (!c.check)
-          flag = 1;
-        else {
-          flag = check_distance(c);
-          if (flag && c.ago == std::max(c.every, c.delay)) c.ndanger++;
-        }
+      const bool due = c.ago >= c.delay && c.ago % c.every == 0;
+      if (due && !c.check)
+        flag = 1;
+      else {
+        flag = check_distance(c, due && c.check, &inner);
+        if (flag && c.ago == std::max(c.every, c.delay)) c.ndanger++;
       }
-      if (flag)
+      if (flag) {
         build_neighbor(c);
-      else
+      } else {
         site_refresh(c);
+        if (inner) build_inner(c);
+      }
       compute_forces(c, evstep ? 1 : 0, evstep ? 1 : 0);
       nve_final(c, dt);
       if (evstep) record();
@@ -782,6 +804,36 @@ int cac_hip_neigh_stats(cac_hip_ctx* h, int* ncalls, int* ndanger) {
   return guarded(h, [&](Ctx& c) -> int {
     if (ncalls) *ncalls = c.ncalls;
     if (ndanger) *ndanger = c.ndanger;
+    return CAC_HIP_OK;
+  });
+}
+
+int cac_hip_inner_stats(cac_hip_ctx* h, long* builds, double* mean_entries, double* delta) {
+  return guarded(h, [&](Ctx& c) -> int {
+    if (builds) *builds = c.inner_builds;
+    if (delta) *delta = c.inner_skin;
+    if (mean_entries) {
+      double m = 0.0;
+      if (c.built && c.P) {
+        std::vector<int> h(c.P);
+        CAC_HIP(hipStreamSynchronize(c.stream));
+        CAC_HIP(hipMemcpy(h.data(), c.innum.get(), sizeof(int) * c.P, hipMemcpyDeviceToHost));
+        long s = 0;
+        for (int v : h) s += v;
+        m = (double)s / c.P;
+      }
+      *mean_entries = m;
+    }
+    return CAC_HIP_OK;
+  });
+}
+
+int cac_hip_set_inner_skin(cac_hip_ctx* h, double delta) {
+  return guarded(h, [&](Ctx& c) -> int {
+    if (!(delta == delta)) return fail(h, CAC_HIP_EARG, "inner skin must be a number");
+    c.inner_skin_set = delta;
+    neighbor_init(c);
+    c.inner_mode = -1;  // re-prune at the next force call
     return CAC_HIP_OK;
   });
 }

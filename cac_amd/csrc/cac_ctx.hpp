@@ -7,12 +7,19 @@
 //           (Hex node order).  x/v/f are [3][P] planes; f holds atom forces and
 //           assembled nodal forces (nodef).  gf [3][8E] is the Gauss-cell force at
 //           each node (2x2x2 Gauss cells sit on the nodes).
-//   sites   S = P + needed virtual atoms, sorted by cell.  sx [3][S] is the site
-//           position wrapped into the box; sdesc names the source (point index >= 0,
-//           or -(1 + e*nucell + u) for unit cell u of element e); simg is the box
-//           shift applied when wrapping; sfp is F'(rho) at the site (EAM).
-//   rows    R = P points in cell order; nlist is column-major [cap][R] of
-//           (site | image << 27), image = (tx+1) + 3(ty+1) + 9(tz+1).
+//   sites   S = P + the virtual atoms some list references, in cell order.  site[S]
+//           is a 32-byte record (x, y, z, F'(rho)): the position wrapped into the
+//           box and, for EAM, the embedding derivative at the site, so one pair
+//           needs one 32-byte gather.  sdesc names the source (point index >= 0, or
+//           -(1 + e*nucell + u) for unit cell u of element e); simg is the box
+//           shift applied when wrapping.
+//   rows    R = P points in cell order; nlist is row-major [R][cap] of unsigned
+//           (site | image << 27), image = (tx+1) + 3(ty+1) + 9(tz+1): every site
+//           image within rc + skin (the reference's list).  ilist [R][icap] is the
+//           inner list the force passes walk: the entries within rc + delta,
+//           pruned on device and rebuilt whenever a point has moved delta/2
+//           since (FP64: same entry format; mixed: site | cell offset << 25 into
+//           site32, FP32 positions relative to each site's cell corner).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -60,6 +67,14 @@ struct DBuf {
     n = c;
   }
   T* get() const { return p; }
+  void swap(DBuf& o) {
+    T* tp = p;
+    p = o.p;
+    o.p = tp;
+    size_t tn = n;
+    n = o.n;
+    o.n = tn;
+  }
 };
 
 struct Timer {
@@ -72,8 +87,9 @@ struct ErrorFlags {  // device-written status words
   int deform;        // element deformed > 2x
   int overflow;      // neighbor row over capacity
   int maxcount;      // largest row seen in the last build
-  int rebuild;       // check_distance result
-  int pad[3];
+  int rebuild;       // check_distance result (outer list)
+  int rebuild_in;    // inner list no longer covers rc
+  int pad[2];
 };
 
 struct Ctx {
@@ -122,12 +138,26 @@ struct Ctx {
   int nc[3] = {0, 0, 0};
   long ncells = 0;
   double cinv[3] = {0, 0, 0};
-  DBuf<double> sx, sfp, tsx;
-  DBuf<float> sfp32;
-  DBuf<int> sdesc, simg, tdesc, timg, tcell, perm, point_site, row_point, row_site, ecount, eoffset, scan_tmp;
+  DBuf<double4> site, site2;
+  DBuf<float4> site32;  // mixed path: (x, y, z) - cell corner, F'(rho)
+  DBuf<int> scell, scell2;  // cell of each site (the frame of site32)
+  DBuf<double> tsx;
+  DBuf<int> sdesc, simg, sdesc2, simg2, used, newidx;
+  DBuf<int> point_site, estart, eslot, ecur;  // site of each point; live virtual sites of each element (CSR)
+  DBuf<int> tdesc, timg, tcell, perm, row_point, row_site, ecount, eoffset, scan_tmp;
   DBuf<int> cell_count, cell_start, cell_cursor, occ, nearA, nearB;
   DBuf<int> nnum, nlist;
   int cap = 0;
+  DBuf<int> innum, ilist;
+  int icap = 0;
+  double inner_skin = 0.0;       // delta in use
+  double inner_skin_set = -1.0;  // requested delta; < 0 picks 0.25 skin
+  double inner_cutsq = 0.0;
+  int inner_mode = -1;       // precision the inner list was built for (-1: stale)
+  long inner_builds = 0;
+  int inner_due = 0;         // set by cac_hip_neigh_decide
+  bool x_dirty = false;      // points moved since the sites were last refreshed
+  DBuf<double> xhold_in;
   long entries = 0;
   DBuf<int> isflag, rowidx;
 
@@ -144,8 +174,11 @@ struct Ctx {
   int nrho = 0, nr = 0;
   double drho = 0, dr = 0, rdr = 0, rdrho = 0, rhomax = 0, cutmax = 0, cutforcesq = 0;
   DBuf<double> frho_spl, rhor_spl, z2r_spl;  // [(n+1)*7]
-  DBuf<float> frho_spl32, rhor_spl32, z2r_spl32;
+  DBuf<float> tab32;   // per knot: rhor c3..c6, z2r c3..c6 (FP32, mixed path)
+  DBuf<double> tab64;  // the same in FP64
   int precision = 0;     // 0 FP64, 1 mixed
+  int tpr = 16;          // lanes per neighbor row in the pair kernels
+  int grid_cap = 2048;   // max workgroups of the persistent pair kernels
   int nodef_style = 0;   // 0 consistent, 1 lumped, 2 node/only
 
   // energy / virial
@@ -169,7 +202,8 @@ struct Ctx {
 void launch_pbc(Ctx& c);
 void check_deformation(Ctx& c, bool first);
 void build_neighbor(Ctx& c);  // pbc + sites + cells + full lists
-int check_distance(Ctx& c);   // returns 1 when any point moved more than skin/2
+int check_distance(Ctx& c, bool outer, int* inner);  // 1 when any point moved more than skin/2
+void build_inner(Ctx& c);     // prune the outer list to rc + delta for the force passes
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
                   std::vector<int>& ucell, std::vector<int>& img);
 void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total);

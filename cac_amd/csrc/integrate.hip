@@ -65,6 +65,7 @@ void nve_initial(Ctx& c, double dt) {
   const double dtf = 0.5 * dt * c.ftm2v;
   k_nve_initial<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.v.get(), c.f.get(), c.ptype.get(), c.P,
                                                           mass_tab(c), dtf, dt);
+  c.x_dirty = true;
   if (c.E) update_centers(c);
 }
 

@@ -374,18 +374,18 @@ __global__ void k_cell_sort(const int* __restrict__ start, const int* __restrict
 }
 
 __global__ void k_apply_perm(const int* __restrict__ perm, int S, int S_cap, const double* __restrict__ tsx,
-                             const int* __restrict__ tdesc, const int* __restrict__ timg, double* __restrict__ sx,
-                             int* __restrict__ sdesc, int* __restrict__ simg, int* __restrict__ point_site,
-                             int* __restrict__ isrow) {
+                             const int* __restrict__ tdesc, const int* __restrict__ timg,
+                             const int* __restrict__ tcell, double4* __restrict__ site, int* __restrict__ sdesc,
+                             int* __restrict__ simg, int* __restrict__ scell, int* __restrict__ isrow) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S) return;
   const int s = perm[k];
-  for (int d = 0; d < 3; d++) sx[(size_t)d * S_cap + k] = tsx[(size_t)d * S_cap + s];
+  scell[k] = tcell[s];
+  site[k] = make_double4(tsx[s], tsx[(size_t)S_cap + s], tsx[2 * (size_t)S_cap + s], 0.0);
   const int desc = tdesc[s];
   sdesc[k] = desc;
   simg[k] = timg[s];
   isrow[k] = desc >= 0;
-  if (desc >= 0) point_site[desc] = k;
 }
 
 __global__ void k_rows(const int* __restrict__ sdesc, const int* __restrict__ isrow, const int* __restrict__ rowidx,
@@ -400,22 +400,23 @@ __global__ void k_rows(const int* __restrict__ sdesc, const int* __restrict__ is
 // One thread per row: scan the 5x5x5 block of cells around the row's cell
 // (cell edge > (rc+skin)/2, periodic), keep every site image within rc + skin
 // except the row's own site, append in scan order (deterministic).
-__global__ __launch_bounds__(256) void k_build_lists(const double* __restrict__ sx, int S_cap,
-                                                     const int* __restrict__ row_site, int R, Grid g,
-                                                     const int* __restrict__ cstart, double cutsq,
-                                                     int* __restrict__ nnum, int* __restrict__ nlist, int cap,
+__global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__ site, const int* __restrict__ row_site,
+                                                     int R, Grid g, const int* __restrict__ cstart, double cutsq,
+                                                     int* __restrict__ nnum, unsigned* __restrict__ nlist, int cap,
                                                      ErrorFlags* flags) {
 #pragma clang fp contract(off)
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   const int k = row_site[r];
-  const double px = sx[k], py = sx[(size_t)S_cap + k], pz = sx[2 * (size_t)S_cap + k];
+  const double4 pk = site[k];
+  const double px = pk.x, py = pk.y, pz = pk.z;
   int ic[3];
   const double pp[3] = {px, py, pz};
   for (int d = 0; d < 3; d++) {
     int v = (int)floor((pp[d] - g.lo[d]) * g.cinv[d]);
     ic[d] = v < 0 ? 0 : (v >= g.nc[d] ? g.nc[d] - 1 : v);
   }
+  unsigned* row = nlist + (size_t)r * cap;
   int n = 0;
   for (int dz = -2; dz <= 2; dz++) {
     int cz = ic[2] + dz, tz = 0;
@@ -432,16 +433,17 @@ __global__ __launch_bounds__(256) void k_build_lists(const double* __restrict__ 
         if (cx < 0) { cx += g.nc[0]; tx = -1; }
         else if (cx >= g.nc[0]) { cx -= g.nc[0]; tx = 1; }
         const double shx = (double)tx * g.L[0];
-        const int tcode = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+        const unsigned tcode = (unsigned)((tx + 1) + 3 * (ty + 1) + 9 * (tz + 1));
         const long cid = ((long)cz * g.nc[1] + cy) * g.nc[0] + cx;
         const int j0 = cstart[cid], j1 = cstart[cid + 1];
         for (int j = j0; j < j1; j++) {
-          const double delx = px - (sx[j] + shx);
-          const double dely = py - (sx[(size_t)S_cap + j] + shy);
-          const double delz = pz - (sx[2 * (size_t)S_cap + j] + shz);
+          const double4 sj = site[j];
+          const double delx = px - (sj.x + shx);
+          const double dely = py - (sj.y + shy);
+          const double delz = pz - (sj.z + shz);
           const double rsq = delx * delx + dely * dely + delz * delz;
-          if (rsq <= cutsq && !(j == k && tcode == 13)) {
-            if (n < cap) nlist[(size_t)n * R + r] = (int)((unsigned)j | ((unsigned)tcode << IMG_SHIFT));
+          if (rsq <= cutsq && !(j == k && tcode == 13u)) {
+            if (n < cap) row[n] = (unsigned)j | (tcode << IMG_SHIFT);
             n++;
           }
         }
@@ -451,6 +453,67 @@ __global__ __launch_bounds__(256) void k_build_lists(const double* __restrict__ 
   nnum[r] = n < cap ? n : cap;
   atomicMax(&flags->maxcount, n);
   if (n > cap) atomicOr(&flags->overflow, 1);
+}
+
+// keep only the sites some row references (and every row's own site)
+__global__ void k_mark_used(const int* __restrict__ row_site, const int* __restrict__ nnum,
+                            const unsigned* __restrict__ nlist, int R, int cap, int* __restrict__ used) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  used[row_site[r]] = 1;
+  const unsigned* row = nlist + (size_t)r * cap;
+  const int n = nnum[r];
+  for (int i = 0; i < n; i++) used[row[i] & SITE_MASK] = 1;
+}
+
+__global__ void k_compact_sites(const int* __restrict__ used, const int* __restrict__ newidx, int S,
+                                const double4* __restrict__ site, const int* __restrict__ sdesc,
+                                const int* __restrict__ simg, const int* __restrict__ scell,
+                                double4* __restrict__ site2, int* __restrict__ sdesc2, int* __restrict__ simg2,
+                                int* __restrict__ scell2) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S || !used[k]) return;
+  const int d = newidx[k];
+  site2[d] = site[k];
+  sdesc2[d] = sdesc[k];
+  simg2[d] = simg[k];
+  scell2[d] = scell[k];
+}
+
+__global__ void k_remap_rows(int* __restrict__ row_site, const int* __restrict__ nnum, unsigned* __restrict__ nlist,
+                             int R, int cap, const int* __restrict__ newidx) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  row_site[r] = newidx[row_site[r]];
+  unsigned* row = nlist + (size_t)r * cap;
+  const int n = nnum[r];
+  for (int i = 0; i < n; i++) {
+    const unsigned e = row[i];
+    row[i] = (unsigned)newidx[e & SITE_MASK] | (e & ~SITE_MASK);
+  }
+}
+
+// per-point site index and the per-element CSR of live interpolated sites, so
+// the per-step refresh and embedding passes read each element's nodes once
+__global__ void k_site_maps(const int* __restrict__ sdesc, int S, int nucell, int* __restrict__ point_site,
+                            int* __restrict__ ecount) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S) return;
+  const int d = sdesc[k];
+  if (d >= 0)
+    point_site[d] = k;
+  else
+    atomicAdd(&ecount[(-d - 1) / nucell], 1);
+}
+
+__global__ void k_elem_fill(const int* __restrict__ sdesc, int S, int nucell, const int* __restrict__ estart,
+                            int* __restrict__ ecur, int* __restrict__ eslot) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S) return;
+  const int d = sdesc[k];
+  if (d >= 0) return;
+  const int e = (-d - 1) / nucell;
+  eslot[estart[e] + atomicAdd(&ecur[e], 1)] = k;
 }
 
 static void read_flags(Ctx& c) {
@@ -529,18 +592,16 @@ void build_neighbor(Ctx& c) {
                                                          c.perm.get());
   k_cell_sort<<<grid_for(c.ncells, 256), 256, 0, c.stream>>>(c.cell_start.get(), c.cell_count.get(), c.ncells,
                                                               c.perm.get());
-  c.sx.reserve((size_t)3 * S_cap);
-  c.sfp.reserve(S_cap);
-  c.sfp32.reserve(S_cap);
+  c.site.reserve(S_cap);
   c.sdesc.reserve(S_cap);
   c.simg.reserve(S_cap);
   c.isflag.reserve(S_cap);
-  c.point_site.reserve(P);
   c.row_point.reserve(P);
   c.row_site.reserve(P);
+  c.scell.reserve(S_cap);
   k_apply_perm<<<grid_for(S, 256), 256, 0, c.stream>>>(c.perm.get(), S, S_cap, c.tsx.get(), c.tdesc.get(),
-                                                       c.timg.get(), c.sx.get(), c.sdesc.get(), c.simg.get(),
-                                                       c.point_site.get(), c.isflag.get());
+                                                       c.timg.get(), c.tcell.get(), c.site.get(), c.sdesc.get(),
+                                                       c.simg.get(), c.scell.get(), c.isflag.get());
   c.rowidx.reserve(S);
   int nrows = 0;
   scan_exclusive(c, c.isflag.get(), c.rowidx.get(), S, &nrows);
@@ -549,6 +610,7 @@ void build_neighbor(Ctx& c) {
                                                  c.row_point.get(), c.row_site.get());
   c.S = S;
   c.nvirt = nvirt;
+  (void)S_cap;
 
   // full lists; grow the per-point capacity and rebuild once if a row overflowed
   if (c.cap == 0) c.cap = 64;
@@ -556,9 +618,10 @@ void build_neighbor(Ctx& c) {
   for (int attempt = 0; attempt < 3; attempt++) {
     c.nlist.reserve((size_t)c.cap * P);
     CAC_HIP(hipMemsetAsync(&c.d_flags.get()->overflow, 0, sizeof(int) * 2, c.stream));
-    k_build_lists<<<grid_for(P, 256), 256, 0, c.stream>>>(c.sx.get(), S_cap, c.row_site.get(), P, g,
-                                                          c.cell_start.get(), c.cutneighsq, c.nnum.get(),
-                                                          c.nlist.get(), c.cap, c.d_flags.get());
+    k_build_lists<<<grid_for(P, 256), 256, 0, c.stream>>>(c.site.get(), c.row_site.get(), P, g, c.cell_start.get(),
+                                                          c.cutneighsq, c.nnum.get(),
+                                                          reinterpret_cast<unsigned*>(c.nlist.get()), c.cap,
+                                                          c.d_flags.get());
     read_flags(c);
     if (!c.h_flags->overflow) break;
     int need = c.h_flags->maxcount;
@@ -568,6 +631,53 @@ void build_neighbor(Ctx& c) {
   if (c.h_flags->nan & 1) throw HipError{hipErrorInvalidValue, "Non-numeric positions - simulation unstable"};
   if (c.h_flags->nan & 2) throw HipError{hipErrorInvalidValue, "an element is larger than the periodic box"};
   if (c.h_flags->deform) throw HipError{hipErrorInvalidValue, "Simulation unstable! an element deformed by more than 2.0 times"};
+  // drop the sites no list references: the per-step refresh and the gathers
+  // then touch only live sites (order, and so cell locality, is kept)
+  {
+    c.used.reserve(S);
+    c.newidx.reserve(S);
+    CAC_HIP(hipMemsetAsync(c.used.get(), 0, sizeof(int) * S, c.stream));
+    unsigned* nl = reinterpret_cast<unsigned*>(c.nlist.get());
+    k_mark_used<<<grid_for(P, 256), 256, 0, c.stream>>>(c.row_site.get(), c.nnum.get(), nl, P, c.cap, c.used.get());
+    int S2 = 0;
+    scan_exclusive(c, c.used.get(), c.newidx.get(), S, &S2);
+    c.site2.reserve(S2);
+    c.sdesc2.reserve(S2);
+    c.simg2.reserve(S2);
+    c.scell2.reserve(S2);
+    k_compact_sites<<<grid_for(S, 256), 256, 0, c.stream>>>(c.used.get(), c.newidx.get(), S, c.site.get(),
+                                                            c.sdesc.get(), c.simg.get(), c.scell.get(),
+                                                            c.site2.get(), c.sdesc2.get(), c.simg2.get(),
+                                                            c.scell2.get());
+    k_remap_rows<<<grid_for(P, 256), 256, 0, c.stream>>>(c.row_site.get(), c.nnum.get(), nl, P, c.cap,
+                                                         c.newidx.get());
+    c.site.swap(c.site2);
+    c.sdesc.swap(c.sdesc2);
+    c.simg.swap(c.simg2);
+    c.scell.swap(c.scell2);
+    c.S = S2;
+    c.nvirt = S2 - P;
+  }
+  {
+    c.point_site.reserve(P);
+    const int E = c.E;
+    if (E) {
+      c.ecount.reserve(E + 1);
+      c.estart.reserve(E + 1);
+      c.ecur.reserve(E + 1);
+      CAC_HIP(hipMemsetAsync(c.ecount.get(), 0, sizeof(int) * (E + 1), c.stream));
+      CAC_HIP(hipMemsetAsync(c.ecur.get(), 0, sizeof(int) * (E + 1), c.stream));
+    }
+    k_site_maps<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.sdesc.get(), c.S, c.nucell > 0 ? c.nucell : 1,
+                                                          c.point_site.get(), E ? c.ecount.get() : nullptr);
+    if (E) {
+      int tot = 0;
+      scan_exclusive(c, c.ecount.get(), c.estart.get(), E + 1, &tot);
+      c.eslot.reserve(tot > 0 ? tot : 1);
+      k_elem_fill<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.sdesc.get(), c.S, c.nucell, c.estart.get(),
+                                                            c.ecur.get(), c.eslot.get());
+    }
+  }
   // entries actually stored
   {
     std::vector<int> h(P);
@@ -577,31 +687,156 @@ void build_neighbor(Ctx& c) {
     c.entries = s;
   }
   c.built = true;
+  c.inner_mode = -1;
+  site_refresh(c);  // fills the mixed path's cell-relative copy
+  build_inner(c);
 }
 
 // ------------------------------------------------------------------ check_distance (neighbor.cpp:1617-1683)
+// One pass tests both lists: the reference's trigger (any point moved more
+// than skin/2 since the last build) and the inner list's (moved more than
+// delta/2 since it was pruned).  A virtual atom moves by a convex combination
+// of its element's node moves, so points bound every site.
 
-__global__ void k_check_distance(const double* __restrict__ x, const double* __restrict__ xh, int P, double trig,
+__global__ void k_check_distance(const double* __restrict__ x, const double* __restrict__ xh,
+                                 const double* __restrict__ xhi, int P, double trig, double trig_in, int outer,
                                  ErrorFlags* flags) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  int hit = 0;
+  int hit = 0, hit_in = 0;
   if (p < P) {
-    double dx = x[p] - xh[p];
-    double dy = x[(size_t)P + p] - xh[(size_t)P + p];
-    double dz = x[2 * (size_t)P + p] - xh[2 * (size_t)P + p];
-    hit = (dx * dx + dy * dy + dz * dz > trig);
+    const double x0 = x[p], x1 = x[(size_t)P + p], x2 = x[2 * (size_t)P + p];
+    if (outer) {
+      const double dx = x0 - xh[p], dy = x1 - xh[(size_t)P + p], dz = x2 - xh[2 * (size_t)P + p];
+      hit = (dx * dx + dy * dy + dz * dz > trig);
+    }
+    const double ex = x0 - xhi[p], ey = x1 - xhi[(size_t)P + p], ez = x2 - xhi[2 * (size_t)P + p];
+    hit_in = (ex * ex + ey * ey + ez * ez > trig_in);
   }
-  if (__ballot(hit) && (threadIdx.x & 63) == 0) atomicOr(&flags->rebuild, 1);
+  const bool lead = (threadIdx.x & 63) == 0;
+  if (__ballot(hit) && lead) atomicOr(&flags->rebuild, 1);
+  if (__ballot(hit_in) && lead) atomicOr(&flags->rebuild_in, 1);
 }
 
-int check_distance(Ctx& c) {
-  CAC_HIP(hipMemsetAsync(&c.d_flags.get()->rebuild, 0, sizeof(int), c.stream));
-  k_check_distance<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.xhold.get(), c.P, c.triggersq,
-                                                             c.d_flags.get());
-  CAC_HIP(hipMemcpyAsync(&c.h_flags->rebuild, &c.d_flags.get()->rebuild, sizeof(int), hipMemcpyDeviceToHost,
+__global__ void k_clear_checks(ErrorFlags* flags) {
+  flags->rebuild = 0;
+  flags->rebuild_in = 0;
+}
+
+int check_distance(Ctx& c, bool outer, int* inner) {
+  k_clear_checks<<<1, 1, 0, c.stream>>>(c.d_flags.get());
+  const double hd = 0.5 * c.inner_skin;
+  k_check_distance<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.xhold.get(), c.xhold_in.get(), c.P,
+                                                             c.triggersq, hd * hd, outer ? 1 : 0, c.d_flags.get());
+  CAC_HIP(hipMemcpyAsync(&c.h_flags->rebuild, &c.d_flags.get()->rebuild, 2 * sizeof(int), hipMemcpyDeviceToHost,
                          c.stream));
   CAC_HIP(hipStreamSynchronize(c.stream));
+  if (inner) *inner = c.h_flags->rebuild_in;
   return c.h_flags->rebuild;
+}
+
+// ------------------------------------------------------------------ inner list
+// 16 lanes per row walk the outer list and keep the entries within rc + delta;
+// a ballot per 16-lane group and a prefix of its bits place them in outer-list
+// order, so the inner list is deterministic.  The mixed path re-encodes each
+// entry as (site | cell offset << 25): the site image sits (dx, dy, dz) cells
+// from the row's cell, which with cell-relative FP32 coordinates gives the
+// pair vector as (u_i - u_j) - d * cell edge.
+
+constexpr int MIX_SHIFT = 25;
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site, const int* __restrict__ scell,
+                                               const int* __restrict__ row_site, const int* __restrict__ nnum,
+                                               const unsigned* __restrict__ nlist, int R, int cap, Grid g,
+                                               double cutsq_in, int* __restrict__ innum, unsigned* __restrict__ ilist,
+                                               int icap, ErrorFlags* flags) {
+  const int lane = threadIdx.x & 15;
+  const int gbase = (threadIdx.x & 63) & ~15;
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
+  const bool active = r < R;
+  const int n = active ? nnum[r] : 0;
+  double4 pk = make_double4(0.0, 0.0, 0.0, 0.0);
+  int icp[3] = {0, 0, 0};
+  if (active) {
+    const int k = row_site[r];
+    pk = site[k];
+    const int cp = scell[k];
+    icp[0] = cp % g.nc[0];
+    icp[1] = (cp / g.nc[0]) % g.nc[1];
+    icp[2] = cp / (g.nc[0] * g.nc[1]);
+  }
+  const unsigned* row = nlist + (size_t)(active ? r : 0) * cap;
+  unsigned* out = ilist + (size_t)(active ? r : 0) * icap;
+  int count = 0;
+  for (int base = 0; base < n; base += 16) {
+    const int jj = base + lane;
+    bool keep = false;
+    unsigned eout = 0u;
+    if (jj < n) {
+      const unsigned ent = row[jj];
+      const int j = (int)(ent & SITE_MASK);
+      const unsigned t = ent >> IMG_SHIFT;
+      const int tx = (int)(t % 3u) - 1, ty = (int)((t / 3u) % 3u) - 1, tz = (int)(t / 9u) - 1;
+      const double4 sj = site[j];
+      const double dx = pk.x - (sj.x + (double)tx * g.L[0]);
+      const double dy = pk.y - (sj.y + (double)ty * g.L[1]);
+      const double dz = pk.z - (sj.z + (double)tz * g.L[2]);
+      keep = dx * dx + dy * dy + dz * dz <= cutsq_in;
+      if (MIXED) {
+        const int cj = scell[j];
+        const int jx = cj % g.nc[0], jy = (cj / g.nc[0]) % g.nc[1], jz = cj / (g.nc[0] * g.nc[1]);
+        const int ox = jx + tx * g.nc[0] - icp[0];
+        const int oy = jy + ty * g.nc[1] - icp[1];
+        const int oz = jz + tz * g.nc[2] - icp[2];
+        const unsigned dcode = (unsigned)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
+        eout = (unsigned)j | (dcode << MIX_SHIFT);
+      } else {
+        eout = ent;
+      }
+    }
+    const unsigned long long m = __ballot(keep);
+    const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
+    const int rank = __popc(gm & ((1u << lane) - 1u));
+    if (keep && count + rank < icap) out[count + rank] = eout;
+    count += __popc(gm);
+  }
+  if (active && lane == 0) {
+    innum[r] = count < icap ? count : icap;
+    atomicMax(&flags->maxcount, count);
+    if (count > icap) atomicOr(&flags->overflow, 1);
+  }
+}
+
+void build_inner(Ctx& c) {
+  ScopedTimer tm(c, "inner_build");
+  const bool mixed = c.precision == 1;
+  if (mixed && (long)c.S >= (1L << MIX_SHIFT)) throw HipError{hipErrorInvalidValue, "more than 2^25 sites for the mixed list"};
+  const double rin = c.cutforce + c.inner_skin;
+  c.inner_cutsq = rin * rin;
+  Grid g = grid_of(c);
+  if (c.icap == 0) c.icap = 64;
+  c.innum.reserve(c.P);
+  for (int attempt = 0; attempt < 3; attempt++) {
+    c.ilist.reserve((size_t)c.icap * c.P);
+    CAC_HIP(hipMemsetAsync(&c.d_flags.get()->overflow, 0, sizeof(int) * 2, c.stream));
+    const int blocks = grid_for(16L * c.P, 256);
+    const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
+    unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());
+    if (mixed)
+      k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.nnum.get(), nl,
+                                                   c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il, c.icap,
+                                                   c.d_flags.get());
+    else
+      k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.nnum.get(), nl,
+                                                    c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il, c.icap,
+                                                    c.d_flags.get());
+    read_flags(c);
+    if (!c.h_flags->overflow) break;
+    c.icap = ((c.h_flags->maxcount + 15) / 16) * 16 + 16;
+  }
+  CAC_HIP(hipMemcpyAsync(c.xhold_in.get(), c.x.get(), sizeof(double) * 3 * c.P, hipMemcpyDeviceToDevice, c.stream));
+  c.inner_mode = c.precision;
+  c.inner_builds++;
 }
 
 // ------------------------------------------------------------------ export for parity checks
@@ -636,7 +871,7 @@ void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::ve
     int sp[3];
     decode(simg[rs[r]], sp);
     for (int n = 0; n < nnum[r]; n++) {
-      const unsigned ent = (unsigned)nl[(size_t)n * P + r];
+      const unsigned ent = (unsigned)nl[(size_t)r * c.cap + n];
       const int j = (int)(ent & SITE_MASK), t = (int)(ent >> IMG_SHIFT);
       int ts[3], ss[3];
       decode(t, ts);

@@ -7,7 +7,7 @@
 //
 // Every point (atom or Gauss point) owns a row of the full neighbor list and
 // sums its own density, force, energy and virial from that row: no scatter, no
-// float atomics, fixed summation order, so results are bit-reproducible.  The
+// float atomics, a fixed summation tree, so results are bit-reproducible.  The
 // reference's half list stores a pair once and applies the reaction to j only
 // when j is an atom or a Gauss point; the full list yields the same forces,
 // because every site that would receive a reaction is itself a row, and sites
@@ -15,68 +15,186 @@
 // virials carry the reference's per-side scale (1/2 for an atom, nodal_weight/2
 // for a node, 0 otherwise; pair_eam.cpp:867-918, 979-984).
 //
+// Work layout: TPR lanes share a row; lane l takes entries l, l+TPR, ... of the
+// row-major list, so one gather instruction of a wavefront touches few cache
+// lines (consecutive entries are sites of the same cells), and the lanes'
+// partial sums are combined with a fixed xor-shuffle tree.  Workgroups are
+// persistent and XCD-aware: workgroup b runs on XCD b % 8 and walks a
+// contiguous eighth of the cell-ordered rows, so neighbouring rows share that
+// XCD's L2.
+//
 // Mixed precision: distances come from FP64 positions and are rounded to FP32;
-// r, the spline evaluations and the pair term run in FP32; density, force,
-// energy and virial accumulate in FP64.
+// r, the spline evaluations (FP32 tables staged in LDS) and the pair term run
+// in FP32; density, force, energy and virial accumulate in FP64.
 #include "cac_ctx.hpp"
 
 namespace cac {
 
+constexpr int PB = 512;  // threads per workgroup of the pair kernels
+constexpr int NXCD = 8;
+
 struct PairArgs {
-  const double* sx;    // [3][S_cap]
+  const double4* site;
+  const float4* site32;
   const int* row_site;
   const int* row_point;
   const int* nnum;
-  const int* nlist;
-  int R, S_cap, A;
+  const unsigned* nlist;
+  int R, cap, A;
   double L[3];
+  float cs[3];
   double cutsq;
+  long nchunks;
 };
 
-__device__ __forceinline__ void img_shift(int code, const double* L, double& sx, double& sy, double& sz) {
-  sx = (double)(code % 3 - 1) * L[0];
-  sy = (double)((code / 3) % 3 - 1) * L[1];
-  sz = (double)(code / 9 - 1) * L[2];
+constexpr int MIX_SHIFT = 25;
+constexpr unsigned MIX_MASK = (1u << MIX_SHIFT) - 1u;
+
+// mixed-path pair vector: site image j lies d = (dx, dy, dz) cells from row i's
+// cell, so x_i - x_j = (u_i - u_j) - d * cell edge with u relative to each
+// site's cell corner
+__device__ __forceinline__ void mixed_delta(unsigned ent, const float4& pi, const float4& sj, const float* cs,
+                                            float& dx, float& dy, float& dz) {
+  const unsigned dc = ent >> MIX_SHIFT;
+  const float ox = (float)((int)(dc % 5u) - 2), oy = (float)((int)((dc / 5u) % 5u) - 2),
+              oz = (float)((int)(dc / 25u) - 2);
+  dx = (pi.x - sj.x) - ox * cs[0];
+  dy = (pi.y - sj.y) - oy * cs[1];
+  dz = (pi.z - sj.z) - oz * cs[2];
+}
+
+__device__ __forceinline__ void img_shift(unsigned code, const double* L, double& sx, double& sy, double& sz) {
+  sx = (double)((int)(code % 3u) - 1) * L[0];
+  sy = (double)((int)((code / 3u) % 3u) - 1) * L[1];
+  sz = (double)((int)(code / 9u) - 1) * L[2];
+}
+
+// XCD-aware persistent schedule: chunk c of rows is handled by the workgroups
+// of XCD x = c / per_xcd, in order.
+struct Sched {
+  long first, end, stride;
+};
+
+__device__ __forceinline__ Sched sched(long nchunks) {
+  const int nwg = gridDim.x;
+  const int xcd = blockIdx.x % NXCD;
+  const int wpx = nwg / NXCD;
+  const int idx = blockIdx.x / NXCD;
+  const long per = (nchunks + NXCD - 1) / NXCD;
+  Sched s;
+  s.first = (long)xcd * per + idx;
+  s.end = (long)(xcd + 1) * per < nchunks ? (long)(xcd + 1) * per : nchunks;
+  s.stride = wpx;
+  return s;
+}
+
+template <int TPR, typename T>
+__device__ __forceinline__ T row_sum(T v) {
+#pragma unroll
+  for (int off = TPR / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, TPR);
+  return v;
 }
 
 // ------------------------------------------------------------------ site refresh
 // forward_comm equivalent: every site position from its source point or from
-// the shape-function interpolation of its element's nodes, plus its wrap shift.
-__global__ void k_site_refresh(const double* __restrict__ x, const double* __restrict__ shape,
-                               const int* __restrict__ sdesc, const int* __restrict__ simg, int S, int S_cap, int A,
-                               int P, int nucell, double L0, double L1, double L2, double* __restrict__ sx) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S) return;
-  const int d = sdesc[k];
-  const int code = simg[k];
-  const double L[3] = {L0, L1, L2};
-  double sh[3];
-  img_shift(code, L, sh[0], sh[1], sh[2]);
-  if (d >= 0) {
-    for (int c = 0; c < 3; c++) sx[(size_t)c * S_cap + k] = x[(size_t)c * P + d] + sh[c];
-  } else {
-    const int q = -d - 1;
-    const int e = q / nucell, u = q - e * nucell;
-    const double* N = shape + (size_t)u * NPE;
-    const size_t base = (size_t)A + (size_t)NPE * e;
-    for (int c = 0; c < 3; c++) {
-      const double* nx = x + (size_t)c * P + base;
-      double s = 0.0;
-      for (int n = 0; n < NPE; n++) s += N[n] * nx[n];
-      sx[(size_t)c * S_cap + k] = s + sh[c];
-    }
+// the shape-function interpolation of its element's nodes, plus its wrap
+// shift.  Points: one thread each.  Interpolated atoms: one wavefront per
+// element, which loads the element's 8 nodes once and walks its live sites.
+
+struct Frame {
+  double L[3], lo[3], cs[3];
+  int nc0, nc1;
+};
+
+__device__ __forceinline__ void write_site(double4* site, float4* site32, const int* scell, const Frame& fr, int k,
+                                           const double* y) {
+  double* out = reinterpret_cast<double*>(site + k);
+  out[0] = y[0];
+  out[1] = y[1];
+  out[2] = y[2];
+  if (site32) {
+    const int cc = scell[k];
+    const int ix = cc % fr.nc0, iy = (cc / fr.nc0) % fr.nc1, iz = cc / (fr.nc0 * fr.nc1);
+    float* o32 = reinterpret_cast<float*>(site32 + k);
+    o32[0] = (float)(y[0] - (fr.lo[0] + ix * fr.cs[0]));
+    o32[1] = (float)(y[1] - (fr.lo[1] + iy * fr.cs[1]));
+    o32[2] = (float)(y[2] - (fr.lo[2] + iz * fr.cs[2]));
   }
+}
+
+__global__ void k_refresh_points(const double* __restrict__ x, const int* __restrict__ point_site,
+                                 const int* __restrict__ simg, const int* __restrict__ scell, int P, Frame fr,
+                                 double4* __restrict__ site, float4* __restrict__ site32) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int k = point_site[p];
+  double sh[3];
+  img_shift((unsigned)simg[k], fr.L, sh[0], sh[1], sh[2]);
+  double y[3];
+  for (int c = 0; c < 3; c++) y[c] = x[(size_t)c * P + p] + sh[c];
+  write_site(site, site32, scell, fr, k, y);
+}
+
+__global__ __launch_bounds__(64) void k_refresh_elements(const double* __restrict__ x,
+                                                         const double* __restrict__ shape,
+                                                         const int* __restrict__ estart,
+                                                         const int* __restrict__ eslot, const int* __restrict__ sdesc,
+                                                         const int* __restrict__ simg, const int* __restrict__ scell,
+                                                         int A, int P, int nucell, Frame fr, double4* __restrict__ site,
+                                                         float4* __restrict__ site32) {
+  const int e = blockIdx.x;
+  __shared__ double nx[3][NPE];
+  if (threadIdx.x < 3 * NPE) {
+    const int c = threadIdx.x / NPE, n = threadIdx.x % NPE;
+    nx[c][n] = x[(size_t)c * P + A + (size_t)NPE * e + n];
+  }
+  __syncthreads();
+  const int i1 = estart[e + 1];
+  for (int i = estart[e] + threadIdx.x; i < i1; i += blockDim.x) {
+    const int k = eslot[i];
+    const int u = (-sdesc[k] - 1) - e * nucell;
+    const double* N = shape + (size_t)u * NPE;
+    double sh[3];
+    img_shift((unsigned)simg[k], fr.L, sh[0], sh[1], sh[2]);
+    double y[3];
+    for (int c = 0; c < 3; c++) {
+      double s = 0.0;
+      for (int n = 0; n < NPE; n++) s += N[n] * nx[c][n];
+      y[c] = s + sh[c];
+    }
+    write_site(site, site32, scell, fr, k, y);
+  }
+}
+
+static Frame frame_of(const Ctx& c) {
+  Frame fr;
+  for (int d = 0; d < 3; d++) {
+    fr.L[d] = c.L[d];
+    fr.lo[d] = c.lo[d];
+    fr.cs[d] = c.L[d] / c.nc[d];
+  }
+  fr.nc0 = c.nc[0];
+  fr.nc1 = c.nc[1];
+  return fr;
 }
 
 void site_refresh(Ctx& c) {
   ScopedTimer tm(c, "site_refresh");
-  k_site_refresh<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.x.get(), c.shape.get(), c.sdesc.get(), c.simg.get(),
-                                                           c.S, c.S, c.A, c.P, c.nucell, c.L[0], c.L[1], c.L[2],
-                                                           c.sx.get());
+  c.x_dirty = false;
+  const bool mixed = c.precision == 1;
+  if (mixed) c.site32.reserve(c.S);
+  const Frame fr = frame_of(c);
+  float4* s32 = mixed ? c.site32.get() : nullptr;
+  k_refresh_points<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.point_site.get(), c.simg.get(),
+                                                             c.scell.get(), c.P, fr, c.site.get(), s32);
+  if (c.E && c.nvirt > 0)
+    k_refresh_elements<<<c.E, 64, 0, c.stream>>>(c.x.get(), c.shape.get(), c.estart.get(), c.eslot.get(),
+                                                 c.sdesc.get(), c.simg.get(), c.scell.get(), c.A, c.P, c.nucell, fr,
+                                                 c.site.get(), s32);
 }
 
 // ------------------------------------------------------------------ spline helpers
-// knot index and fraction exactly as pair_eam.cpp:799-805 / 1160-1169
+// knot index and fraction as pair_eam.cpp:799-805 and 1160-1169
 
 template <typename T>
 __device__ __forceinline__ int knot_r(T r, T rdr, int nr, T& p) {
@@ -99,47 +217,104 @@ __device__ __forceinline__ int knot_rho(double rho, double rdrho, int nrho, doub
   return m;
 }
 
+// spline table in LDS: knot m holds (rhor c3..c6) at 2m and (z2r c3..c6) at 2m+1,
+// float4 for the mixed path, double4 for FP64; derivatives are formed from
+// these value coefficients, ((3 c3 p + 2 c4) p + c5) / dr, which is what the
+// reference's coefficients c0..c2 hold
+template <typename V>
+__device__ __forceinline__ void stage_table(V* lds, const V* __restrict__ g, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ row walker
+// Lane l of a row's TPR lanes visits entries l, l + TPR, ...; U entries at a
+// time: U list loads, then U site gathers, then the U pair terms, so U
+// independent memory chains are in flight per lane instead of one.
+
+constexpr int UNROLL = 4;
+
+template <int TPR, int U, typename Site, typename Fetch, typename Body>
+__device__ __forceinline__ void walk_row(const unsigned* __restrict__ row, int n, int lane, unsigned self,
+                                         Fetch fetch, Body body) {
+  for (int j0 = lane; j0 < n; j0 += U * TPR) {
+    unsigned e[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int jj = j0 + u * TPR;
+      e[u] = jj < n ? row[jj] : self;
+    }
+    Site sj[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) sj[u] = fetch(e[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (j0 + u * TPR < n) body(e[u], sj[u]);
+  }
+}
+
 // ------------------------------------------------------------------ EAM density (pair_eam.cpp:721-813)
 
-template <bool MIXED>
-__global__ __launch_bounds__(256) void k_eam_density(PairArgs a, const double* __restrict__ rhor,
-                                                     const float* __restrict__ rhor32, double rdr, int nr,
-                                                     double* __restrict__ rho_out) {
-#pragma clang fp contract(off)
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.R) return;
-  const int k = a.row_site[r];
-  const double px = a.sx[k], py = a.sx[(size_t)a.S_cap + k], pz = a.sx[2 * (size_t)a.S_cap + k];
-  const int n = a.nnum[r];
-  double rho = 0.0;
-  for (int jj = 0; jj < n; jj++) {
-    const unsigned ent = (unsigned)a.nlist[(size_t)jj * a.R + r];
-    const int j = (int)(ent & SITE_MASK);
-    double shx, shy, shz;
-    img_shift((int)(ent >> IMG_SHIFT), a.L, shx, shy, shz);
-    const double delx = px - (a.sx[j] + shx);
-    const double dely = py - (a.sx[(size_t)a.S_cap + j] + shy);
-    const double delz = pz - (a.sx[2 * (size_t)a.S_cap + j] + shz);
-    if (MIXED) {
-      const float fx = (float)delx, fy = (float)dely, fz = (float)delz;
-      const float rsq = fx * fx + fy * fy + fz * fz;
-      if (rsq < (float)a.cutsq) {
-        float p;
-        const int m = knot_r<float>(sqrtf(rsq), (float)rdr, nr, p);
-        const float* c = rhor32 + (size_t)m * 4;  // value coefficients 3..6
-        rho += (double)(((c[0] * p + c[1]) * p + c[2]) * p + c[3]);
-      }
-    } else {
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < a.cutsq) {
-        double p;
-        const int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
-        const double* c = rhor + (size_t)m * 7;
-        rho += ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+template <int TPR, bool MIXED, int NT>
+__global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __restrict__ tab, double rdr, int nr,
+                                                    double* __restrict__ rho_out) {
+  extern __shared__ float4 lds_raw[];
+  float4* lds32 = lds_raw;
+  double4* lds64 = reinterpret_cast<double4*>(lds_raw);
+  if (MIXED)
+    stage_table(lds32, static_cast<const float4*>(tab), 2 * (nr + 1));
+  else
+    stage_table(lds64, static_cast<const double4*>(tab), 2 * (nr + 1));
+  const int lane = threadIdx.x % TPR;
+  const int rsub = threadIdx.x / TPR;
+  const Sched s = sched(a.nchunks);
+  const float rdr32 = (float)rdr;
+  const float cut32 = (float)a.cutsq;
+  for (long ch = s.first; ch < s.end; ch += s.stride) {
+    const int r = (int)(ch * (NT / TPR)) + rsub;
+    double acc = 0.0;
+    if (r < a.R) {
+      const int k = a.row_site[r];
+      const int n = a.nnum[r];
+      const unsigned* row = a.nlist + (size_t)r * a.cap;
+      if (MIXED) {
+        const float4 pi = a.site32[k];
+        walk_row<TPR, UNROLL, float4>(
+            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
+            [&](unsigned ent, const float4& sj) {
+              float dx, dy, dz;
+              mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
+              const float rsq = dx * dx + dy * dy + dz * dz;
+              if (rsq < cut32) {
+                float p;
+                const int m = knot_r<float>(__fsqrt_rn(rsq), rdr32, nr, p);
+                const float4 c = lds32[2 * m];
+                acc += (double)(((c.x * p + c.y) * p + c.z) * p + c.w);
+              }
+            });
+      } else {
+        const double4 pi = a.site[k];
+        walk_row<TPR, UNROLL, double4>(
+            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
+            [&](unsigned ent, const double4& sj) {
+              double shx, shy, shz;
+              img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
+              const double delx = pi.x - (sj.x + shx);
+              const double dely = pi.y - (sj.y + shy);
+              const double delz = pi.z - (sj.z + shz);
+              const double rsq = delx * delx + dely * dely + delz * delz;
+              if (rsq < a.cutsq) {
+                double p;
+                const int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
+                const double4 c = lds64[2 * m];
+                acc += ((c.x * p + c.y) * p + c.z) * p + c.w;
+              }
+            });
       }
     }
+    acc = row_sum<TPR>(acc);
+    if (lane == 0 && r < a.R) rho_out[a.row_point[r]] = acc;
   }
-  rho_out[a.row_point[r]] = rho;
 }
 
 // ------------------------------------------------------------------ embedding derivative per site
@@ -147,158 +322,205 @@ __global__ __launch_bounds__(256) void k_eam_density(PairArgs a, const double* _
 // interpolated atoms: F'(sum_k N_k rho_node_k), the value the reference
 // recomputes for every pair (pair_eam.cpp:959-969)
 
-__global__ void k_eam_embed(const double* __restrict__ rho, const double* __restrict__ shape,
-                            const int* __restrict__ sdesc, int S, int A, int nucell,
-                            const double* __restrict__ frho, double rdrho, int nrho, double* __restrict__ sfp,
-                            float* __restrict__ sfp32) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S) return;
-  const int d = sdesc[k];
-  double rj;
-  if (d >= 0) {
-    rj = rho[d];
-  } else {
-    const int q = -d - 1;
-    const int e = q / nucell, u = q - e * nucell;
-    const double* N = shape + (size_t)u * NPE;
-    const double* nr = rho + (size_t)A + (size_t)NPE * e;
-    rj = 0.0;
-    for (int n = 0; n < NPE; n++) rj += N[n] * nr[n];
-  }
+__device__ __forceinline__ double fprime(const double* __restrict__ frho, double rho, double rdrho, int nrho) {
   double p;
-  const int m = knot_rho(rj, rdrho, nrho, p);
+  const int m = knot_rho(rho, rdrho, nrho, p);
   const double* c = frho + (size_t)m * 7;
-  const double fp = (c[0] * p + c[1]) * p + c[2];
-  sfp[k] = fp;
-  sfp32[k] = (float)fp;
+  return (c[0] * p + c[1]) * p + c[2];
+}
+
+__global__ void k_embed_points(const double* __restrict__ rho, const int* __restrict__ point_site, int P,
+                               const double* __restrict__ frho, double rdrho, int nrho, double4* __restrict__ site,
+                               float4* __restrict__ site32) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int k = point_site[p];
+  const double fp = fprime(frho, rho[p], rdrho, nrho);
+  reinterpret_cast<double*>(site + k)[3] = fp;
+  if (site32) reinterpret_cast<float*>(site32 + k)[3] = (float)fp;
+}
+
+__global__ __launch_bounds__(64) void k_embed_elements(const double* __restrict__ rho,
+                                                       const double* __restrict__ shape,
+                                                       const int* __restrict__ estart, const int* __restrict__ eslot,
+                                                       const int* __restrict__ sdesc, int A, int nucell,
+                                                       const double* __restrict__ frho, double rdrho, int nrho,
+                                                       double4* __restrict__ site, float4* __restrict__ site32) {
+  const int e = blockIdx.x;
+  __shared__ double nr[NPE];
+  if (threadIdx.x < NPE) nr[threadIdx.x] = rho[(size_t)A + (size_t)NPE * e + threadIdx.x];
+  __syncthreads();
+  const int i1 = estart[e + 1];
+  for (int i = estart[e] + threadIdx.x; i < i1; i += blockDim.x) {
+    const int k = eslot[i];
+    const int u = (-sdesc[k] - 1) - e * nucell;
+    const double* N = shape + (size_t)u * NPE;
+    double rj = 0.0;
+    for (int n = 0; n < NPE; n++) rj += N[n] * nr[n];
+    const double fp = fprime(frho, rj, rdrho, nrho);
+    reinterpret_cast<double*>(site + k)[3] = fp;
+    if (site32) reinterpret_cast<float*>(site32 + k)[3] = (float)fp;
+  }
 }
 
 // ------------------------------------------------------------------ EAM force (pair_eam.cpp:853-1027)
 
 struct EamForceArgs {
-  const double* rhor;  // [(nr+1)*7]
-  const double* z2r;
-  const double* frho;
-  const float* fr32;   // [(nr+1)*7]: rhor' coeffs 0..2, z2r coeffs 0..6 ... packed per knot
+  const void* tab;     // LDS-staged value coefficients (float4 or double4 pairs per knot)
+  const double* frho;  // [(nrho+1)*7]
   const double* rho;   // per point
-  const double* sfp;
-  const float* sfp32;
   double rdr, rdrho, rhomax;
   int nr, nrho;
-  double escale_atom, escale_node, vscale_node;
+  double escale_node, vscale_node;
 };
 
-template <bool MIXED, bool EV>
-__global__ __launch_bounds__(256) void k_eam_force(PairArgs a, EamForceArgs q, double* __restrict__ f,
-                                                   double* __restrict__ gf, int P, int E8,
-                                                   double* __restrict__ erow) {
-#pragma clang fp contract(off)
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.R) return;
-  const int k = a.row_site[r];
-  const int pnt = a.row_point[r];
-  const bool is_atom = pnt < a.A;
-  const double px = a.sx[k], py = a.sx[(size_t)a.S_cap + k], pz = a.sx[2 * (size_t)a.S_cap + k];
-  const double fpi = q.sfp[k];
-  const float fpi32 = q.sfp32[k];
-  const int n = a.nnum[r];
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  double ephi = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
-  for (int jj = 0; jj < n; jj++) {
-    const unsigned ent = (unsigned)a.nlist[(size_t)jj * a.R + r];
-    const int j = (int)(ent & SITE_MASK);
-    double shx, shy, shz;
-    img_shift((int)(ent >> IMG_SHIFT), a.L, shx, shy, shz);
-    const double delx = px - (a.sx[j] + shx);
-    const double dely = py - (a.sx[(size_t)a.S_cap + j] + shy);
-    const double delz = pz - (a.sx[2 * (size_t)a.S_cap + j] + shz);
-    if (MIXED) {
-      const float dx = (float)delx, dy = (float)dely, dz = (float)delz;
-      const float rsq = dx * dx + dy * dy + dz * dz;
-      if (rsq < (float)a.cutsq) {
-        const float rr = sqrtf(rsq);
-        float p;
-        const int m = knot_r<float>(rr, (float)q.rdr, q.nr, p);
-        const float* c = q.fr32 + (size_t)m * 10;
-        const float rhop = (c[0] * p + c[1]) * p + c[2];
-        const float z2p = (c[3] * p + c[4]) * p + c[5];
-        const float z2 = ((c[6] * p + c[7]) * p + c[8]) * p + c[9];
-        const float recip = 1.0f / rr;
-        const float phi = z2 * recip;
-        const float phip = z2p * recip - phi * recip;
-        const float psip = fpi32 * rhop + q.sfp32[j] * rhop + phip;
-        const float fpair = -psip * recip;
-        fx += (double)(dx * fpair);
-        fy += (double)(dy * fpair);
-        fz += (double)(dz * fpair);
-        if (EV) {
-          ephi += (double)phi;
-          v0 += (double)(dx * dx * fpair);
-          v1 += (double)(dy * dy * fpair);
-          v2 += (double)(dz * dz * fpair);
-          v3 += (double)(dx * dy * fpair);
-          v4 += (double)(dx * dz * fpair);
-          v5 += (double)(dy * dz * fpair);
-        }
-      }
-    } else {
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < a.cutsq) {
-        const double rr = sqrt(rsq);
-        double p;
-        const int m = knot_r<double>(rr, q.rdr, q.nr, p);
-        const double* c = q.rhor + (size_t)m * 7;
-        const double rhop = (c[0] * p + c[1]) * p + c[2];
-        const double* z = q.z2r + (size_t)m * 7;
-        const double z2p = (z[0] * p + z[1]) * p + z[2];
-        const double z2 = ((z[3] * p + z[4]) * p + z[5]) * p + z[6];
-        const double recip = 1.0 / rr;
-        const double phi = z2 * recip;
-        const double phip = z2p * recip - phi * recip;
-        const double psip = fpi * rhop + q.sfp[j] * rhop + phip;
-        const double fpair = -1.0 * psip * recip;
-        fx += delx * fpair;
-        fy += dely * fpair;
-        fz += delz * fpair;
-        if (EV) {
-          ephi += phi;
-          v0 += delx * delx * fpair;
-          v1 += dely * dely * fpair;
-          v2 += delz * delz * fpair;
-          v3 += delx * dely * fpair;
-          v4 += delx * delz * fpair;
-          v5 += dely * delz * fpair;
-        }
+template <int TPR, bool MIXED, bool EV, int NT>
+__global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, double* __restrict__ f,
+                                                  double* __restrict__ gf, int P, int E8,
+                                                  double* __restrict__ erow) {
+  extern __shared__ float4 lds_raw[];
+  float4* lds32 = lds_raw;
+  double4* lds64 = reinterpret_cast<double4*>(lds_raw);
+  if (MIXED)
+    stage_table(lds32, static_cast<const float4*>(q.tab), 2 * (q.nr + 1));
+  else
+    stage_table(lds64, static_cast<const double4*>(q.tab), 2 * (q.nr + 1));
+  const int lane = threadIdx.x % TPR;
+  const int rsub = threadIdx.x / TPR;
+  const Sched s = sched(a.nchunks);
+  const float rdr32 = (float)q.rdr;
+  for (long ch = s.first; ch < s.end; ch += s.stride) {
+    const int r = (int)(ch * (NT / TPR)) + rsub;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double ephi = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+    int k = 0;
+    if (r < a.R) {
+      k = a.row_site[r];
+      const int n = a.nnum[r];
+      const unsigned* row = a.nlist + (size_t)r * a.cap;
+      if (MIXED) {
+        const float4 pi = a.site32[k];
+        const float cut32 = (float)a.cutsq;
+        walk_row<TPR, UNROLL, float4>(
+            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
+            [&](unsigned ent, const float4& sj) {
+              float dx, dy, dz;
+              mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
+              const float rsq = dx * dx + dy * dy + dz * dz;
+              if (rsq < cut32) {
+                const float rr = __fsqrt_rn(rsq);
+                float p;
+                const int m = knot_r<float>(rr, rdr32, q.nr, p);
+                const float4 cr = lds32[2 * m];
+                const float4 cz = lds32[2 * m + 1];
+                const float rhop = ((3.0f * cr.x * p + 2.0f * cr.y) * p + cr.z) * rdr32;
+                const float z2p = ((3.0f * cz.x * p + 2.0f * cz.y) * p + cz.z) * rdr32;
+                const float z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
+                const float recip = 1.0f / rr;
+                const float phi = z2 * recip;
+                const float phip = z2p * recip - phi * recip;
+                const float psip = (pi.w + sj.w) * rhop + phip;
+                const float fpair = -psip * recip;
+                fx += (double)(dx * fpair);
+                fy += (double)(dy * fpair);
+                fz += (double)(dz * fpair);
+                if (EV) {
+                  ephi += (double)phi;
+                  v0 += (double)(dx * dx * fpair);
+                  v1 += (double)(dy * dy * fpair);
+                  v2 += (double)(dz * dz * fpair);
+                  v3 += (double)(dx * dy * fpair);
+                  v4 += (double)(dx * dz * fpair);
+                  v5 += (double)(dy * dz * fpair);
+                }
+              }
+            });
+      } else {
+        const double4 pi = a.site[k];
+        walk_row<TPR, UNROLL, double4>(
+            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
+            [&](unsigned ent, const double4& sj) {
+              double shx, shy, shz;
+              img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
+              const double delx = pi.x - (sj.x + shx);
+              const double dely = pi.y - (sj.y + shy);
+              const double delz = pi.z - (sj.z + shz);
+              const double rsq = delx * delx + dely * dely + delz * delz;
+              if (rsq < a.cutsq) {
+                const double rr = sqrt(rsq);
+                double p;
+                const int m = knot_r<double>(rr, q.rdr, q.nr, p);
+                const double4 cr = lds64[2 * m];
+                const double4 cz = lds64[2 * m + 1];
+                const double rhop = ((3.0 * cr.x * p + 2.0 * cr.y) * p + cr.z) * q.rdr;
+                const double z2p = ((3.0 * cz.x * p + 2.0 * cz.y) * p + cz.z) * q.rdr;
+                const double z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
+                const double recip = 1.0 / rr;
+                const double phi = z2 * recip;
+                const double phip = z2p * recip - phi * recip;
+                const double psip = pi.w * rhop + sj.w * rhop + phip;
+                const double fpair = -1.0 * psip * recip;
+                fx += delx * fpair;
+                fy += dely * fpair;
+                fz += delz * fpair;
+                if (EV) {
+                  ephi += phi;
+                  v0 += delx * delx * fpair;
+                  v1 += dely * dely * fpair;
+                  v2 += delz * delz * fpair;
+                  v3 += delx * dely * fpair;
+                  v4 += delx * delz * fpair;
+                  v5 += dely * delz * fpair;
+                }
+              }
+            });
       }
     }
-  }
-  if (is_atom) {
-    f[pnt] = fx;
-    f[(size_t)P + pnt] = fy;
-    f[2 * (size_t)P + pnt] = fz;
-  } else {
-    const int g = pnt - a.A;
-    gf[g] = fx;
-    gf[(size_t)E8 + g] = fy;
-    gf[2 * (size_t)E8 + g] = fz;
-  }
-  if (EV) {
-    // embedding energy F(rho) at atoms (weight 1) and nodes (weight nodal_weight)
-    const double rho = q.rho[pnt];
-    double p;
-    const int m = knot_rho(rho, q.rdrho, q.nrho, p);
-    const double* c = q.frho + (size_t)m * 7;
-    double phi = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
-    if (rho > q.rhomax) phi += fpi * (rho - q.rhomax);
-    const double es = is_atom ? q.escale_atom : q.escale_node;
-    const double vs = is_atom ? 0.5 : q.vscale_node;
-    erow[r] = phi * (is_atom ? 1.0 : 2.0 * q.escale_node) + es * ephi;
-    erow[(size_t)a.R * 1 + r] = vs * v0;
-    erow[(size_t)a.R * 2 + r] = vs * v1;
-    erow[(size_t)a.R * 3 + r] = vs * v2;
-    erow[(size_t)a.R * 4 + r] = vs * v3;
-    erow[(size_t)a.R * 5 + r] = vs * v4;
-    erow[(size_t)a.R * 6 + r] = vs * v5;
+    fx = row_sum<TPR>(fx);
+    fy = row_sum<TPR>(fy);
+    fz = row_sum<TPR>(fz);
+    if (EV) {
+      ephi = row_sum<TPR>(ephi);
+      v0 = row_sum<TPR>(v0);
+      v1 = row_sum<TPR>(v1);
+      v2 = row_sum<TPR>(v2);
+      v3 = row_sum<TPR>(v3);
+      v4 = row_sum<TPR>(v4);
+      v5 = row_sum<TPR>(v5);
+    }
+    if (lane == 0 && r < a.R) {
+      const int pnt = a.row_point[r];
+      const bool is_atom = pnt < a.A;
+      if (is_atom) {
+        f[pnt] = fx;
+        f[(size_t)P + pnt] = fy;
+        f[2 * (size_t)P + pnt] = fz;
+      } else {
+        const int g = pnt - a.A;
+        gf[g] = fx;
+        gf[(size_t)E8 + g] = fy;
+        gf[2 * (size_t)E8 + g] = fz;
+      }
+      if (EV) {
+        // embedding energy F(rho) at atoms (weight 1) and nodes (weight nodal_weight)
+        const double rho = q.rho[pnt];
+        const double fpi = a.site[k].w;
+        double p;
+        const int m = knot_rho(rho, q.rdrho, q.nrho, p);
+        const double* c = q.frho + (size_t)m * 7;
+        double phi = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+        if (rho > q.rhomax) phi += fpi * (rho - q.rhomax);
+        const double es = is_atom ? 0.5 : q.escale_node;
+        const double vs = is_atom ? 0.5 : q.vscale_node;
+        erow[r] = phi * (is_atom ? 1.0 : 2.0 * q.escale_node) + es * ephi;
+        erow[(size_t)a.R * 1 + r] = vs * v0;
+        erow[(size_t)a.R * 2 + r] = vs * v1;
+        erow[(size_t)a.R * 3 + r] = vs * v2;
+        erow[(size_t)a.R * 4 + r] = vs * v3;
+        erow[(size_t)a.R * 5 + r] = vs * v4;
+        erow[(size_t)a.R * 6 + r] = vs * v5;
+      }
+    }
   }
 }
 
@@ -308,185 +530,251 @@ struct LjArgs {
   double lj1, lj2, lj3, lj4, escale_node, vscale_node;
 };
 
-template <bool MIXED, bool EV>
-__global__ __launch_bounds__(256) void k_lj_force(PairArgs a, LjArgs q, double* __restrict__ f,
-                                                  double* __restrict__ gf, int P, int E8,
-                                                  double* __restrict__ erow) {
-#pragma clang fp contract(off)
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.R) return;
-  const int k = a.row_site[r];
-  const int pnt = a.row_point[r];
-  const bool is_atom = pnt < a.A;
-  const double px = a.sx[k], py = a.sx[(size_t)a.S_cap + k], pz = a.sx[2 * (size_t)a.S_cap + k];
-  const int n = a.nnum[r];
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  double ev = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+template <int TPR, bool MIXED, bool EV>
+__global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* __restrict__ f,
+                                                 double* __restrict__ gf, int P, int E8, double* __restrict__ erow) {
+  const int lane = threadIdx.x % TPR;
+  const int rsub = threadIdx.x / TPR;
+  const Sched s = sched(a.nchunks);
   const float l1 = (float)q.lj1, l2 = (float)q.lj2, l3 = (float)q.lj3, l4 = (float)q.lj4;
-  for (int jj = 0; jj < n; jj++) {
-    const unsigned ent = (unsigned)a.nlist[(size_t)jj * a.R + r];
-    const int j = (int)(ent & SITE_MASK);
-    double shx, shy, shz;
-    img_shift((int)(ent >> IMG_SHIFT), a.L, shx, shy, shz);
-    const double delx = px - (a.sx[j] + shx);
-    const double dely = py - (a.sx[(size_t)a.S_cap + j] + shy);
-    const double delz = pz - (a.sx[2 * (size_t)a.S_cap + j] + shz);
-    if (MIXED) {
-      const float dx = (float)delx, dy = (float)dely, dz = (float)delz;
-      const float rsq = dx * dx + dy * dy + dz * dz;
-      if (rsq < (float)a.cutsq) {
-        const float r2inv = 1.0f / rsq;
-        const float r6inv = r2inv * r2inv * r2inv;
-        const float forcelj = r6inv * (l1 * r6inv - l2);
-        const float fpair = forcelj * r2inv;
-        fx += (double)(dx * fpair);
-        fy += (double)(dy * fpair);
-        fz += (double)(dz * fpair);
-        if (EV) {
-          ev += (double)(r6inv * (l3 * r6inv - l4));
-          v0 += (double)(dx * dx * fpair);
-          v1 += (double)(dy * dy * fpair);
-          v2 += (double)(dz * dz * fpair);
-          v3 += (double)(dx * dy * fpair);
-          v4 += (double)(dx * dz * fpair);
-          v5 += (double)(dy * dz * fpair);
-        }
-      }
-    } else {
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < a.cutsq) {
-        const double r2inv = 1.0 / rsq;
-        const double r6inv = r2inv * r2inv * r2inv;
-        const double forcelj = r6inv * (q.lj1 * r6inv - q.lj2);
-        const double fpair = forcelj * r2inv;
-        fx += delx * fpair;
-        fy += dely * fpair;
-        fz += delz * fpair;
-        if (EV) {
-          ev += r6inv * (q.lj3 * r6inv - q.lj4);
-          v0 += delx * delx * fpair;
-          v1 += dely * dely * fpair;
-          v2 += delz * delz * fpair;
-          v3 += delx * dely * fpair;
-          v4 += delx * delz * fpair;
-          v5 += dely * delz * fpair;
-        }
+  for (long ch = s.first; ch < s.end; ch += s.stride) {
+    const int r = (int)(ch * (PB / TPR)) + rsub;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double ev = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+    if (r < a.R) {
+      const int k = a.row_site[r];
+      const int n = a.nnum[r];
+      const unsigned* row = a.nlist + (size_t)r * a.cap;
+      if (MIXED) {
+        const float4 pi = a.site32[k];
+        const float cut32 = (float)a.cutsq;
+        walk_row<TPR, UNROLL, float4>(
+            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
+            [&](unsigned ent, const float4& sj) {
+              float dx, dy, dz;
+              mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
+              const float rsq = dx * dx + dy * dy + dz * dz;
+              if (rsq < cut32) {
+                const float r2inv = 1.0f / rsq;
+                const float r6inv = r2inv * r2inv * r2inv;
+                const float forcelj = r6inv * (l1 * r6inv - l2);
+                const float fpair = forcelj * r2inv;
+                fx += (double)(dx * fpair);
+                fy += (double)(dy * fpair);
+                fz += (double)(dz * fpair);
+                if (EV) {
+                  ev += (double)(r6inv * (l3 * r6inv - l4));
+                  v0 += (double)(dx * dx * fpair);
+                  v1 += (double)(dy * dy * fpair);
+                  v2 += (double)(dz * dz * fpair);
+                  v3 += (double)(dx * dy * fpair);
+                  v4 += (double)(dx * dz * fpair);
+                  v5 += (double)(dy * dz * fpair);
+                }
+              }
+            });
+      } else {
+        const double4 pi = a.site[k];
+        walk_row<TPR, UNROLL, double4>(
+            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
+            [&](unsigned ent, const double4& sj) {
+              double shx, shy, shz;
+              img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
+              const double delx = pi.x - (sj.x + shx);
+              const double dely = pi.y - (sj.y + shy);
+              const double delz = pi.z - (sj.z + shz);
+              const double rsq = delx * delx + dely * dely + delz * delz;
+              if (rsq < a.cutsq) {
+                const double r2inv = 1.0 / rsq;
+                const double r6inv = r2inv * r2inv * r2inv;
+                const double forcelj = r6inv * (q.lj1 * r6inv - q.lj2);
+                const double fpair = forcelj * r2inv;
+                fx += delx * fpair;
+                fy += dely * fpair;
+                fz += delz * fpair;
+                if (EV) {
+                  ev += r6inv * (q.lj3 * r6inv - q.lj4);
+                  v0 += delx * delx * fpair;
+                  v1 += dely * dely * fpair;
+                  v2 += delz * delz * fpair;
+                  v3 += delx * dely * fpair;
+                  v4 += delx * delz * fpair;
+                  v5 += dely * delz * fpair;
+                }
+              }
+            });
       }
     }
-  }
-  if (is_atom) {
-    f[pnt] = fx;
-    f[(size_t)P + pnt] = fy;
-    f[2 * (size_t)P + pnt] = fz;
-  } else {
-    const int g = pnt - a.A;
-    gf[g] = fx;
-    gf[(size_t)E8 + g] = fy;
-    gf[2 * (size_t)E8 + g] = fz;
-  }
-  if (EV) {
-    const double es = is_atom ? 0.5 : q.escale_node;
-    const double vs = is_atom ? 0.5 : q.vscale_node;
-    erow[r] = es * ev;
-    erow[(size_t)a.R * 1 + r] = vs * v0;
-    erow[(size_t)a.R * 2 + r] = vs * v1;
-    erow[(size_t)a.R * 3 + r] = vs * v2;
-    erow[(size_t)a.R * 4 + r] = vs * v3;
-    erow[(size_t)a.R * 5 + r] = vs * v4;
-    erow[(size_t)a.R * 6 + r] = vs * v5;
+    fx = row_sum<TPR>(fx);
+    fy = row_sum<TPR>(fy);
+    fz = row_sum<TPR>(fz);
+    if (EV) {
+      ev = row_sum<TPR>(ev);
+      v0 = row_sum<TPR>(v0);
+      v1 = row_sum<TPR>(v1);
+      v2 = row_sum<TPR>(v2);
+      v3 = row_sum<TPR>(v3);
+      v4 = row_sum<TPR>(v4);
+      v5 = row_sum<TPR>(v5);
+    }
+    if (lane == 0 && r < a.R) {
+      const int pnt = a.row_point[r];
+      const bool is_atom = pnt < a.A;
+      if (is_atom) {
+        f[pnt] = fx;
+        f[(size_t)P + pnt] = fy;
+        f[2 * (size_t)P + pnt] = fz;
+      } else {
+        const int g = pnt - a.A;
+        gf[g] = fx;
+        gf[(size_t)E8 + g] = fy;
+        gf[2 * (size_t)E8 + g] = fz;
+      }
+      if (EV) {
+        const double es = is_atom ? 0.5 : q.escale_node;
+        const double vs = is_atom ? 0.5 : q.vscale_node;
+        erow[r] = es * ev;
+        erow[(size_t)a.R * 1 + r] = vs * v0;
+        erow[(size_t)a.R * 2 + r] = vs * v1;
+        erow[(size_t)a.R * 3 + r] = vs * v2;
+        erow[(size_t)a.R * 4 + r] = vs * v3;
+        erow[(size_t)a.R * 5 + r] = vs * v4;
+        erow[(size_t)a.R * 6 + r] = vs * v5;
+      }
+    }
   }
 }
 
 // ------------------------------------------------------------------ orchestration
 
-static PairArgs pair_args(Ctx& c, double cutsq) {
+static PairArgs pair_args(Ctx& c, double cutsq, int rows_per_wg) {
   PairArgs a;
-  a.sx = c.sx.get();
+  a.site = c.site.get();
+  a.site32 = c.site32.get();
   a.row_site = c.row_site.get();
   a.row_point = c.row_point.get();
-  a.nnum = c.nnum.get();
-  a.nlist = c.nlist.get();
+  a.nnum = c.innum.get();
+  a.nlist = reinterpret_cast<const unsigned*>(c.ilist.get());
   a.R = c.P;
-  a.S_cap = c.S;
+  a.cap = c.icap;
   a.A = c.A;
-  for (int d = 0; d < 3; d++) a.L[d] = c.L[d];
+  for (int d = 0; d < 3; d++) {
+    a.L[d] = c.L[d];
+    a.cs[d] = (float)(c.L[d] / c.nc[d]);
+  }
   a.cutsq = cutsq;
+  a.nchunks = (c.P + rows_per_wg - 1) / rows_per_wg;
   return a;
 }
 
-void pair_compute(Ctx& c, int eflag, int vflag) {
-  const int R = c.P;
+static int pair_grid(long nchunks, long cap) {
+  long g = cap < nchunks ? cap : nchunks;
+  g = ((g + NXCD - 1) / NXCD) * NXCD;
+  return (int)(g < NXCD ? NXCD : g);
+}
+
+// kernels staging more than 64 KB of LDS must say so before their first launch
+template <typename K>
+static void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) CAC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+template <int TPR>
+static void launch_pair(Ctx& c, int eflag, int vflag) {
   const bool ev = eflag || vflag;
   const bool mixed = c.precision == 1;
-  if (ev) c.erow.reserve((size_t)7 * R);
-  const int grid = grid_for(R, 256);
   const int E8 = NPE * c.E;
   double* gf = c.gf.get();
+  double* er = ev ? c.erow.get() : nullptr;
   if (c.pairstyle == 2) {
-    PairArgs a = pair_args(c, c.cutforcesq);
+    constexpr int NT32 = 512, NT64 = 1024;  // 64 KB table: two workgroups per CU; 128 KB: one
+    const int nt = mixed ? NT32 : NT64;
+    PairArgs a = pair_args(c, c.cutforcesq, nt / TPR);
+    const int grid = pair_grid(a.nchunks, mixed ? 512 : 256);
+    const size_t lds = (mixed ? sizeof(float4) : sizeof(double4)) * 2 * (c.nr + 1);
+    const void* tab = mixed ? (const void*)c.tab32.get() : (const void*)c.tab64.get();
     {
       ScopedTimer tm(c, "eam_density");
-      if (mixed)
-        k_eam_density<true><<<grid, 256, 0, c.stream>>>(a, c.rhor_spl.get(), c.rhor_spl32.get(), c.rdr, c.nr,
-                                                        c.rho.get());
-      else
-        k_eam_density<false><<<grid, 256, 0, c.stream>>>(a, c.rhor_spl.get(), c.rhor_spl32.get(), c.rdr, c.nr,
-                                                         c.rho.get());
+      if (mixed) {
+        k_eam_density<TPR, true, NT32><<<grid, NT32, lds, c.stream>>>(a, tab, c.rdr, c.nr, c.rho.get());
+      } else {
+        allow_lds(k_eam_density<TPR, false, NT64>, lds);
+        k_eam_density<TPR, false, NT64><<<grid, NT64, lds, c.stream>>>(a, tab, c.rdr, c.nr, c.rho.get());
+      }
     }
     {
       ScopedTimer tm(c, "eam_embed");
-      k_eam_embed<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.rho.get(), c.shape.get(), c.sdesc.get(), c.S, c.A,
-                                                            c.nucell > 0 ? c.nucell : 1, c.frho_spl.get(), c.rdrho,
-                                                            c.nrho, c.sfp.get(), c.sfp32.get());
+      float4* s32 = mixed ? c.site32.get() : nullptr;
+      k_embed_points<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.rho.get(), c.point_site.get(), c.P,
+                                                               c.frho_spl.get(), c.rdrho, c.nrho, c.site.get(), s32);
+      if (c.E && c.nvirt > 0)
+        k_embed_elements<<<c.E, 64, 0, c.stream>>>(c.rho.get(), c.shape.get(), c.estart.get(), c.eslot.get(),
+                                                   c.sdesc.get(), c.A, c.nucell, c.frho_spl.get(), c.rdrho, c.nrho,
+                                                   c.site.get(), s32);
     }
     EamForceArgs q;
-    q.rhor = c.rhor_spl.get();
-    q.z2r = c.z2r_spl.get();
+    q.tab = tab;
     q.frho = c.frho_spl.get();
-    q.fr32 = c.z2r_spl32.get();
     q.rho = c.rho.get();
-    q.sfp = c.sfp.get();
-    q.sfp32 = c.sfp32.get();
     q.rdr = c.rdr;
     q.rdrho = c.rdrho;
     q.rhomax = c.rhomax;
     q.nr = c.nr;
     q.nrho = c.nrho;
-    q.escale_atom = 0.5;
     q.escale_node = c.nodal_weight / 2.0;
     q.vscale_node = c.nodal_weight / 2.0;
     ScopedTimer tm(c, "eam_force");
     if (mixed) {
       if (ev)
-        k_eam_force<true, true><<<grid, 256, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, c.erow.get());
+        k_eam_force<TPR, true, true, NT32><<<grid, NT32, lds, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
       else
-        k_eam_force<true, false><<<grid, 256, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, nullptr);
+        k_eam_force<TPR, true, false, NT32><<<grid, NT32, lds, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
     } else {
-      if (ev)
-        k_eam_force<false, true><<<grid, 256, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, c.erow.get());
-      else
-        k_eam_force<false, false><<<grid, 256, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, nullptr);
+      if (ev) {
+        allow_lds(k_eam_force<TPR, false, true, NT64>, lds);
+        k_eam_force<TPR, false, true, NT64><<<grid, NT64, lds, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+      } else {
+        allow_lds(k_eam_force<TPR, false, false, NT64>, lds);
+        k_eam_force<TPR, false, false, NT64><<<grid, NT64, lds, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+      }
     }
   } else {
-    PairArgs a = pair_args(c, c.ljcutsq);
+    PairArgs a = pair_args(c, c.ljcutsq, PB / TPR);
+    const int grid = pair_grid(a.nchunks, c.grid_cap);
     LjArgs q{c.lj1, c.lj2, c.lj3, c.lj4, c.nodal_weight / 2.0, c.nodal_weight / 2.0};
     ScopedTimer tm(c, "lj_force");
     if (mixed) {
       if (ev)
-        k_lj_force<true, true><<<grid, 256, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, c.erow.get());
+        k_lj_force<TPR, true, true><<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
       else
-        k_lj_force<true, false><<<grid, 256, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, nullptr);
+        k_lj_force<TPR, true, false><<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
     } else {
       if (ev)
-        k_lj_force<false, true><<<grid, 256, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, c.erow.get());
+        k_lj_force<TPR, false, true><<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
       else
-        k_lj_force<false, false><<<grid, 256, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, nullptr);
+        k_lj_force<TPR, false, false><<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
     }
+  }
+}
+
+void pair_compute(Ctx& c, int eflag, int vflag) {
+  const bool ev = eflag || vflag;
+  if (ev) c.erow.reserve((size_t)7 * c.P);
+  if (c.inner_mode != c.precision) {  // precision changed since the last prune
+    site_refresh(c);
+    build_inner(c);
+  }
+  switch (c.tpr) {
+    case 4: launch_pair<4>(c, eflag, vflag); break;
+    case 8: launch_pair<8>(c, eflag, vflag); break;
+    case 32: launch_pair<32>(c, eflag, vflag); break;
+    default: launch_pair<16>(c, eflag, vflag); break;
   }
   if (c.E) nodal_assembly(c);
   if (ev) {
     double out[7];
-    reduce_rows(c, c.erow.get(), R, 7, out);
+    reduce_rows(c, c.erow.get(), c.P, 7, out);
     if (eflag) c.eng_vdwl = out[0];
     if (vflag)
       for (int i = 0; i < 6; i++) c.virial[i] = out[1 + i];
