@@ -25,21 +25,25 @@ namespace cac {
 Ctx::~Ctx() {
   if (h_red) (void)hipHostFree(h_red);
   if (h_flags) (void)hipHostFree(h_flags);
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
+  for (auto& row : ev)
+    for (auto& e : row)
+      if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 ScopedTimer::ScopedTimer(Ctx& cc, const char* nm) : c(cc), name(nm) {
-  if (c.timing) CAC_HIP(hipEventRecord(c.ev0, c.stream));
+  if (!c.timing || c.tdepth >= 8) return;
+  depth = c.tdepth++;
+  CAC_HIP(hipEventRecord(c.ev[0][depth], c.stream));
 }
 
 ScopedTimer::~ScopedTimer() {
-  if (!c.timing) return;
-  if (hipEventRecord(c.ev1, c.stream) != hipSuccess) return;
-  if (hipEventSynchronize(c.ev1) != hipSuccess) return;
+  if (depth < 0) return;
+  c.tdepth = depth;
+  if (hipEventRecord(c.ev[1][depth], c.stream) != hipSuccess) return;
+  if (hipEventSynchronize(c.ev[1][depth]) != hipSuccess) return;
   float ms = 0.0f;
-  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) != hipSuccess) return;
+  if (hipEventElapsedTime(&ms, c.ev[0][depth], c.ev[1][depth]) != hipSuccess) return;
   Timer& t = c.timers[name];
   t.sum_us += 1000.0 * ms;
   t.count++;
@@ -152,9 +156,9 @@ void compute_forces(Ctx& c, int eflag, int vflag) {
   // the forward comm of a step without rebuild: refresh the sites, and re-prune
   // the inner list if a point has moved half its buffer
   if (c.x_dirty) {
-    site_refresh(c);
     int inner = 0;
     check_distance(c, false, &inner);
+    site_refresh(c, inner != 0);
     if (inner) build_inner(c);
   }
   // force_clear is implicit: every row writes its own force and every element
@@ -182,8 +186,8 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     c.device = device;
     CAC_HIP(hipSetDevice(device));
     CAC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    CAC_HIP(hipEventCreate(&c.ev0));
-    CAC_HIP(hipEventCreate(&c.ev1));
+    for (auto& row : c.ev)
+      for (auto& e : row) CAC_HIP(hipEventCreate(&e));
     CAC_HIP(hipHostMalloc((void**)&c.h_red, sizeof(double) * 64, hipHostMallocDefault));
     CAC_HIP(hipHostMalloc((void**)&c.h_flags, sizeof(ErrorFlags), hipHostMallocDefault));
     c.d_flags.reserve(1);
@@ -558,8 +562,19 @@ int cac_hip_neigh_decide(cac_hip_ctx* h, int* flag) {
   });
 }
 
+static long count_entries(Ctx& c, const DBuf<int>& nnum) {
+  if (!c.built || !c.P) return 0;
+  std::vector<int> h(c.P);
+  CAC_HIP(hipStreamSynchronize(c.stream));
+  CAC_HIP(hipMemcpy(h.data(), nnum.get(), sizeof(int) * c.P, hipMemcpyDeviceToHost));
+  long s = 0;
+  for (int v : h) s += v;
+  return s;
+}
+
 int cac_hip_neigh_info(cac_hip_ctx* h, int* rows, long* entries, int* sites, int* capacity) {
   return guarded(h, [&](Ctx& c) -> int {
+    if (c.entries < 0) c.entries = count_entries(c, c.nnum);
     if (rows) *rows = c.P;
     if (entries) *entries = c.entries;
     if (sites) *sites = c.S;
@@ -573,6 +588,7 @@ int cac_hip_neigh_export(cac_hip_ctx* h, long* n, int* row, int* kind, int* owne
     if (!c.built) return fail(h, CAC_HIP_ESTATE, "neighbor list not built");
     if (!n) return fail(h, CAC_HIP_EARG, "null count");
     if (!row) {
+      if (c.entries < 0) c.entries = count_entries(c, c.nnum);
       *n = c.entries;
       return CAC_HIP_OK;
     }
@@ -692,7 +708,7 @@ int cac_hip_run(cac_hip_ctx* h, int nsteps, double dt, int thermo_every, double*
       if (flag) {
         build_neighbor(c);
       } else {
-        site_refresh(c);
+        site_refresh(c, inner != 0);
         if (inner) build_inner(c);
       }
       compute_forces(c, evstep ? 1 : 0, evstep ? 1 : 0);
@@ -812,18 +828,7 @@ int cac_hip_inner_stats(cac_hip_ctx* h, long* builds, double* mean_entries, doub
   return guarded(h, [&](Ctx& c) -> int {
     if (builds) *builds = c.inner_builds;
     if (delta) *delta = c.inner_skin;
-    if (mean_entries) {
-      double m = 0.0;
-      if (c.built && c.P) {
-        std::vector<int> h(c.P);
-        CAC_HIP(hipStreamSynchronize(c.stream));
-        CAC_HIP(hipMemcpy(h.data(), c.innum.get(), sizeof(int) * c.P, hipMemcpyDeviceToHost));
-        long s = 0;
-        for (int v : h) s += v;
-        m = (double)s / c.P;
-      }
-      *mean_entries = m;
-    }
+    if (mean_entries) *mean_entries = c.P && c.inner_mode >= 0 ? (double)count_entries(c, c.innum) / c.P : 0.0;
     return CAC_HIP_OK;
   });
 }

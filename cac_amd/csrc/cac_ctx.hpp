@@ -34,6 +34,15 @@ namespace cac {
 constexpr int NPE = 8;
 constexpr int IMG_SHIFT = 27;
 constexpr unsigned SITE_MASK = (1u << IMG_SHIFT) - 1u;  // list entry = site | image << 27, read as unsigned
+
+// a site's neighbor cell is kept as packed coordinates ix | iy << 11 | iz << 22
+constexpr int CELL_BITS_X = 11, CELL_BITS_Y = 11, CELL_BITS_Z = 10;
+__host__ __device__ inline int cell_pack(int ix, int iy, int iz) {
+  return ix | (iy << CELL_BITS_X) | (iz << (CELL_BITS_X + CELL_BITS_Y));
+}
+__host__ __device__ inline int cell_ix(int p) { return p & ((1 << CELL_BITS_X) - 1); }
+__host__ __device__ inline int cell_iy(int p) { return (p >> CELL_BITS_X) & ((1 << CELL_BITS_Y) - 1); }
+__host__ __device__ inline int cell_iz(int p) { return (p >> (CELL_BITS_X + CELL_BITS_Y)) & ((1 << CELL_BITS_Z) - 1); }
 constexpr int MAXT = 8;
 
 struct HipError {
@@ -143,7 +152,6 @@ struct Ctx {
   DBuf<int> scell, scell2;  // cell of each site (the frame of site32)
   DBuf<double> tsx;
   DBuf<int> sdesc, simg, sdesc2, simg2, used, newidx;
-  DBuf<int> point_site, estart, eslot, ecur;  // site of each point; live virtual sites of each element (CSR)
   DBuf<int> tdesc, timg, tcell, perm, row_point, row_site, ecount, eoffset, scan_tmp;
   DBuf<int> cell_count, cell_start, cell_cursor, occ, nearA, nearB;
   DBuf<int> nnum, nlist;
@@ -192,7 +200,8 @@ struct Ctx {
   // timing
   bool timing = false;
   std::map<std::string, Timer> timers;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev[2][8] = {};  // start / stop events of nested timers, by depth
+  int tdepth = 0;
 
   ~Ctx();
 };
@@ -209,7 +218,9 @@ void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::ve
 void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total);
 
 // pair.hip
-void site_refresh(Ctx& c);
+// positions of every live site; full = also the FP64 records (always kept in
+// FP64 mode; the mixed path needs them only to build or prune lists)
+void site_refresh(Ctx& c, bool full);
 void pair_compute(Ctx& c, int eflag, int vflag);
 
 // integrate.hip
@@ -221,9 +232,10 @@ void thermo(Ctx& c, double* ke, double* temp);
 void reduce_rows(Ctx& c, const double* data, int n, int ncomp, double* out);
 
 // timing helpers (api.hip)
-struct ScopedTimer {
+struct ScopedTimer {  // GPU time between two events on the context stream; nests up to 8 deep
   Ctx& c;
   const char* name;
+  int depth = -1;
   ScopedTimer(Ctx& cc, const char* nm);
   ~ScopedTimer();
 };

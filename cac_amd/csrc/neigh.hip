@@ -373,14 +373,15 @@ __global__ void k_cell_sort(const int* __restrict__ start, const int* __restrict
   }
 }
 
-__global__ void k_apply_perm(const int* __restrict__ perm, int S, int S_cap, const double* __restrict__ tsx,
+__global__ void k_apply_perm(const int* __restrict__ perm, int S, int S_cap, Grid g, const double* __restrict__ tsx,
                              const int* __restrict__ tdesc, const int* __restrict__ timg,
                              const int* __restrict__ tcell, double4* __restrict__ site, int* __restrict__ sdesc,
                              int* __restrict__ simg, int* __restrict__ scell, int* __restrict__ isrow) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S) return;
   const int s = perm[k];
-  scell[k] = tcell[s];
+  const int cid = tcell[s];
+  scell[k] = cell_pack(cid % g.nc[0], (cid / g.nc[0]) % g.nc[1], cid / (g.nc[0] * g.nc[1]));
   site[k] = make_double4(tsx[s], tsx[(size_t)S_cap + s], tsx[2 * (size_t)S_cap + s], 0.0);
   const int desc = tdesc[s];
   sdesc[k] = desc;
@@ -397,15 +398,24 @@ __global__ void k_rows(const int* __restrict__ sdesc, const int* __restrict__ is
   row_site[r] = k;
 }
 
-// One thread per row: scan the 5x5x5 block of cells around the row's cell
-// (cell edge > (rc+skin)/2, periodic), keep every site image within rc + skin
-// except the row's own site, append in scan order (deterministic).
+// BL lanes per row scan the 5x5x5 block of cells around the row's cell (cell
+// edge > (rc+skin)/2, periodic) and keep every site image within rc + skin
+// except the row's own site.  Sites are sorted by cell, so the five cells of
+// one x strip are one contiguous site range (two across the periodic seam):
+// the lanes read it coalesced, and a ballot + prefix per lane group appends
+// the kept sites in scan order (z, y, x cell order, site order inside a
+// cell), so the list is deterministic.  Only an overflowing row touches the
+// shared status words.
+constexpr int BL = 8;
+
 __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__ site, const int* __restrict__ row_site,
                                                      int R, Grid g, const int* __restrict__ cstart, double cutsq,
                                                      int* __restrict__ nnum, unsigned* __restrict__ nlist, int cap,
                                                      ErrorFlags* flags) {
 #pragma clang fp contract(off)
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x % BL;
+  const int gbase = (threadIdx.x & 63) & ~(BL - 1);
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / BL);
   if (r >= R) return;
   const int k = row_site[r];
   const double4 pk = site[k];
@@ -417,6 +427,9 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
     ic[d] = v < 0 ? 0 : (v >= g.nc[d] ? g.nc[d] - 1 : v);
   }
   unsigned* row = nlist + (size_t)r * cap;
+  const int nx = g.nc[0];
+  const int xlo = ic[0] - 2, xhi = ic[0] + 2;
+  const int nseg = (xlo < 0 || xhi >= nx) ? 2 : 1;
   int n = 0;
   for (int dz = -2; dz <= 2; dz++) {
     int cz = ic[2] + dz, tz = 0;
@@ -428,42 +441,61 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
       if (cy < 0) { cy += g.nc[1]; ty = -1; }
       else if (cy >= g.nc[1]) { cy -= g.nc[1]; ty = 1; }
       const double shy = (double)ty * g.L[1];
-      for (int dx = -2; dx <= 2; dx++) {
-        int cx = ic[0] + dx, tx = 0;
-        if (cx < 0) { cx += g.nc[0]; tx = -1; }
-        else if (cx >= g.nc[0]) { cx -= g.nc[0]; tx = 1; }
+      const long cbase = ((long)cz * g.nc[1] + cy) * nx;
+      for (int sg = 0; sg < nseg; sg++) {
+        // the strip's cells in dx order: the part across the seam first when it lies at -x
+        int a = xlo, b = xhi, tx = 0;
+        if (xlo < 0) {
+          if (sg == 0) { a = xlo + nx; b = nx - 1; tx = -1; }
+          else { a = 0; b = xhi; }
+        } else if (xhi >= nx) {
+          if (sg == 0) { a = xlo; b = nx - 1; }
+          else { a = 0; b = xhi - nx; tx = 1; }
+        }
         const double shx = (double)tx * g.L[0];
         const unsigned tcode = (unsigned)((tx + 1) + 3 * (ty + 1) + 9 * (tz + 1));
-        const long cid = ((long)cz * g.nc[1] + cy) * g.nc[0] + cx;
-        const int j0 = cstart[cid], j1 = cstart[cid + 1];
-        for (int j = j0; j < j1; j++) {
-          const double4 sj = site[j];
-          const double delx = px - (sj.x + shx);
-          const double dely = py - (sj.y + shy);
-          const double delz = pz - (sj.z + shz);
-          const double rsq = delx * delx + dely * dely + delz * delz;
-          if (rsq <= cutsq && !(j == k && tcode == 13u)) {
-            if (n < cap) row[n] = (unsigned)j | (tcode << IMG_SHIFT);
-            n++;
+        const int j0 = cstart[cbase + a], j1 = cstart[cbase + b + 1];
+        for (int jb = j0; jb < j1; jb += BL) {
+          const int j = jb + lane;
+          bool keep = false;
+          if (j < j1) {
+            const double4 sj = site[j];
+            const double delx = px - (sj.x + shx);
+            const double dely = py - (sj.y + shy);
+            const double delz = pz - (sj.z + shz);
+            const double rsq = delx * delx + dely * dely + delz * delz;
+            keep = rsq <= cutsq && !(j == k && tcode == 13u);
           }
+          const unsigned long long m = __ballot(keep);
+          const unsigned gm = (unsigned)((m >> gbase) & ((1ull << BL) - 1ull));
+          const int rank = __popc(gm & ((1u << lane) - 1u));
+          if (keep && n + rank < cap) row[n + rank] = (unsigned)j | (tcode << IMG_SHIFT);
+          n += __popc(gm);
         }
       }
     }
   }
-  nnum[r] = n < cap ? n : cap;
-  atomicMax(&flags->maxcount, n);
-  if (n > cap) atomicOr(&flags->overflow, 1);
+  if (lane == 0) {
+    nnum[r] = n < cap ? n : cap;
+    if (n > cap) {
+      atomicMax(&flags->maxcount, n);
+      atomicOr(&flags->overflow, 1);
+    }
+  }
 }
 
-// keep only the sites some row references (and every row's own site)
-__global__ void k_mark_used(const int* __restrict__ row_site, const int* __restrict__ nnum,
-                            const unsigned* __restrict__ nlist, int R, int cap, int* __restrict__ used) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// keep only the sites some row references (and every row's own site);
+// 16 lanes per row read the row coalesced
+__global__ __launch_bounds__(256) void k_mark_used(const int* __restrict__ row_site, const int* __restrict__ nnum,
+                                                   const unsigned* __restrict__ nlist, int R, int cap,
+                                                   int* __restrict__ used) {
+  const int lane = threadIdx.x & 15;
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
   if (r >= R) return;
-  used[row_site[r]] = 1;
+  if (lane == 0) used[row_site[r]] = 1;
   const unsigned* row = nlist + (size_t)r * cap;
   const int n = nnum[r];
-  for (int i = 0; i < n; i++) used[row[i] & SITE_MASK] = 1;
+  for (int i = lane; i < n; i += 16) used[row[i] & SITE_MASK] = 1;
 }
 
 __global__ void k_compact_sites(const int* __restrict__ used, const int* __restrict__ newidx, int S,
@@ -480,40 +512,19 @@ __global__ void k_compact_sites(const int* __restrict__ used, const int* __restr
   scell2[d] = scell[k];
 }
 
-__global__ void k_remap_rows(int* __restrict__ row_site, const int* __restrict__ nnum, unsigned* __restrict__ nlist,
-                             int R, int cap, const int* __restrict__ newidx) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_remap_rows(int* __restrict__ row_site, const int* __restrict__ nnum,
+                                                    unsigned* __restrict__ nlist, int R, int cap,
+                                                    const int* __restrict__ newidx) {
+  const int lane = threadIdx.x & 15;
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
   if (r >= R) return;
-  row_site[r] = newidx[row_site[r]];
+  if (lane == 0) row_site[r] = newidx[row_site[r]];
   unsigned* row = nlist + (size_t)r * cap;
   const int n = nnum[r];
-  for (int i = 0; i < n; i++) {
+  for (int i = lane; i < n; i += 16) {
     const unsigned e = row[i];
     row[i] = (unsigned)newidx[e & SITE_MASK] | (e & ~SITE_MASK);
   }
-}
-
-// per-point site index and the per-element CSR of live interpolated sites, so
-// the per-step refresh and embedding passes read each element's nodes once
-__global__ void k_site_maps(const int* __restrict__ sdesc, int S, int nucell, int* __restrict__ point_site,
-                            int* __restrict__ ecount) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S) return;
-  const int d = sdesc[k];
-  if (d >= 0)
-    point_site[d] = k;
-  else
-    atomicAdd(&ecount[(-d - 1) / nucell], 1);
-}
-
-__global__ void k_elem_fill(const int* __restrict__ sdesc, int S, int nucell, const int* __restrict__ estart,
-                            int* __restrict__ ecur, int* __restrict__ eslot) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S) return;
-  const int d = sdesc[k];
-  if (d >= 0) return;
-  const int e = (-d - 1) / nucell;
-  eslot[estart[e] + atomicAdd(&ecur[e], 1)] = k;
 }
 
 static void read_flags(Ctx& c) {
@@ -539,7 +550,8 @@ void build_neighbor(Ctx& c) {
     c.cinv[d] = (double)n / c.L[d];
   }
   c.ncells = (long)c.nc[0] * c.nc[1] * c.nc[2];
-  if (c.ncells > (1L << 30)) throw HipError{hipErrorInvalidValue, "too many neighbor cells"};
+  if (c.nc[0] >= (1 << CELL_BITS_X) || c.nc[1] >= (1 << CELL_BITS_Y) || c.nc[2] >= (1 << CELL_BITS_Z))
+    throw HipError{hipErrorInvalidValue, "too many neighbor cells along a box edge"};
   Grid g = grid_of(c);
   const int P = c.P;
 
@@ -599,7 +611,7 @@ void build_neighbor(Ctx& c) {
   c.row_point.reserve(P);
   c.row_site.reserve(P);
   c.scell.reserve(S_cap);
-  k_apply_perm<<<grid_for(S, 256), 256, 0, c.stream>>>(c.perm.get(), S, S_cap, c.tsx.get(), c.tdesc.get(),
+  k_apply_perm<<<grid_for(S, 256), 256, 0, c.stream>>>(c.perm.get(), S, S_cap, g, c.tsx.get(), c.tdesc.get(),
                                                        c.timg.get(), c.tcell.get(), c.site.get(), c.sdesc.get(),
                                                        c.simg.get(), c.scell.get(), c.isflag.get());
   c.rowidx.reserve(S);
@@ -613,12 +625,16 @@ void build_neighbor(Ctx& c) {
   (void)S_cap;
 
   // full lists; grow the per-point capacity and rebuild once if a row overflowed
-  if (c.cap == 0) c.cap = 64;
+  if (c.cap == 0) {  // first build: the mean count at the site density, with margin
+    const double vol = c.L[0] * c.L[1] * c.L[2];
+    const double est = (double)S / vol * 4.18879020478639 * c.cutneighsq * c.cutneigh;
+    c.cap = std::min(c.one, ((int)(1.5 * est) + 32 + 15) / 16 * 16);
+  }
   c.nnum.reserve(P);
   for (int attempt = 0; attempt < 3; attempt++) {
     c.nlist.reserve((size_t)c.cap * P);
     CAC_HIP(hipMemsetAsync(&c.d_flags.get()->overflow, 0, sizeof(int) * 2, c.stream));
-    k_build_lists<<<grid_for(P, 256), 256, 0, c.stream>>>(c.site.get(), c.row_site.get(), P, g, c.cell_start.get(),
+    k_build_lists<<<grid_for((long)BL * P, 256), 256, 0, c.stream>>>(c.site.get(), c.row_site.get(), P, g, c.cell_start.get(),
                                                           c.cutneighsq, c.nnum.get(),
                                                           reinterpret_cast<unsigned*>(c.nlist.get()), c.cap,
                                                           c.d_flags.get());
@@ -638,7 +654,8 @@ void build_neighbor(Ctx& c) {
     c.newidx.reserve(S);
     CAC_HIP(hipMemsetAsync(c.used.get(), 0, sizeof(int) * S, c.stream));
     unsigned* nl = reinterpret_cast<unsigned*>(c.nlist.get());
-    k_mark_used<<<grid_for(P, 256), 256, 0, c.stream>>>(c.row_site.get(), c.nnum.get(), nl, P, c.cap, c.used.get());
+    k_mark_used<<<grid_for(16L * P, 256), 256, 0, c.stream>>>(c.row_site.get(), c.nnum.get(), nl, P, c.cap,
+                                                               c.used.get());
     int S2 = 0;
     scan_exclusive(c, c.used.get(), c.newidx.get(), S, &S2);
     c.site2.reserve(S2);
@@ -649,8 +666,8 @@ void build_neighbor(Ctx& c) {
                                                             c.sdesc.get(), c.simg.get(), c.scell.get(),
                                                             c.site2.get(), c.sdesc2.get(), c.simg2.get(),
                                                             c.scell2.get());
-    k_remap_rows<<<grid_for(P, 256), 256, 0, c.stream>>>(c.row_site.get(), c.nnum.get(), nl, P, c.cap,
-                                                         c.newidx.get());
+    k_remap_rows<<<grid_for(16L * P, 256), 256, 0, c.stream>>>(c.row_site.get(), c.nnum.get(), nl, P, c.cap,
+                                                                c.newidx.get());
     c.site.swap(c.site2);
     c.sdesc.swap(c.sdesc2);
     c.simg.swap(c.simg2);
@@ -658,37 +675,10 @@ void build_neighbor(Ctx& c) {
     c.S = S2;
     c.nvirt = S2 - P;
   }
-  {
-    c.point_site.reserve(P);
-    const int E = c.E;
-    if (E) {
-      c.ecount.reserve(E + 1);
-      c.estart.reserve(E + 1);
-      c.ecur.reserve(E + 1);
-      CAC_HIP(hipMemsetAsync(c.ecount.get(), 0, sizeof(int) * (E + 1), c.stream));
-      CAC_HIP(hipMemsetAsync(c.ecur.get(), 0, sizeof(int) * (E + 1), c.stream));
-    }
-    k_site_maps<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.sdesc.get(), c.S, c.nucell > 0 ? c.nucell : 1,
-                                                          c.point_site.get(), E ? c.ecount.get() : nullptr);
-    if (E) {
-      int tot = 0;
-      scan_exclusive(c, c.ecount.get(), c.estart.get(), E + 1, &tot);
-      c.eslot.reserve(tot > 0 ? tot : 1);
-      k_elem_fill<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.sdesc.get(), c.S, c.nucell, c.estart.get(),
-                                                            c.ecur.get(), c.eslot.get());
-    }
-  }
-  // entries actually stored
-  {
-    std::vector<int> h(P);
-    CAC_HIP(hipMemcpy(h.data(), c.nnum.get(), sizeof(int) * P, hipMemcpyDeviceToHost));
-    long s = 0;
-    for (int v : h) s += v;
-    c.entries = s;
-  }
+  c.entries = -1;  // stored entries, counted on request
   c.built = true;
   c.inner_mode = -1;
-  site_refresh(c);  // fills the mixed path's cell-relative copy
+  site_refresh(c, true);  // fills the mixed path's cell-relative copy
   build_inner(c);
 }
 
@@ -737,10 +727,11 @@ int check_distance(Ctx& c, bool outer, int* inner) {
 // ------------------------------------------------------------------ inner list
 // 16 lanes per row walk the outer list and keep the entries within rc + delta;
 // a ballot per 16-lane group and a prefix of its bits place them in outer-list
-// order, so the inner list is deterministic.  The mixed path re-encodes each
-// entry as (site | cell offset << 25): the site image sits (dx, dy, dz) cells
-// from the row's cell, which with cell-relative FP32 coordinates gives the
-// pair vector as (u_i - u_j) - d * cell edge.
+// order, so the inner list is deterministic.  It has the outer list's row
+// capacity, so it cannot overflow and needs no status check.  The mixed path
+// re-encodes each entry as (site | cell offset << 25): the site image sits
+// (dx, dy, dz) cells from the row's cell, which with cell-relative FP32
+// coordinates gives the pair vector as (u_i - u_j) - d * cell edge.
 
 constexpr int MIX_SHIFT = 25;
 
@@ -748,25 +739,19 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site, const int* __restrict__ scell,
                                                const int* __restrict__ row_site, const int* __restrict__ nnum,
                                                const unsigned* __restrict__ nlist, int R, int cap, Grid g,
-                                               double cutsq_in, int* __restrict__ innum, unsigned* __restrict__ ilist,
-                                               int icap, ErrorFlags* flags) {
+                                               double cutsq_in, int* __restrict__ innum,
+                                               unsigned* __restrict__ ilist) {
   const int lane = threadIdx.x & 15;
   const int gbase = (threadIdx.x & 63) & ~15;
   const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
-  const bool active = r < R;
-  const int n = active ? nnum[r] : 0;
-  double4 pk = make_double4(0.0, 0.0, 0.0, 0.0);
-  int icp[3] = {0, 0, 0};
-  if (active) {
-    const int k = row_site[r];
-    pk = site[k];
-    const int cp = scell[k];
-    icp[0] = cp % g.nc[0];
-    icp[1] = (cp / g.nc[0]) % g.nc[1];
-    icp[2] = cp / (g.nc[0] * g.nc[1]);
-  }
-  const unsigned* row = nlist + (size_t)(active ? r : 0) * cap;
-  unsigned* out = ilist + (size_t)(active ? r : 0) * icap;
+  if (r >= R) return;
+  const int n = nnum[r];
+  const int k = row_site[r];
+  const double4 pk = site[k];
+  const int cp = scell[k];
+  const int icx = cell_ix(cp), icy = cell_iy(cp), icz = cell_iz(cp);
+  const unsigned* row = nlist + (size_t)r * cap;
+  unsigned* out = ilist + (size_t)r * cap;
   int count = 0;
   for (int base = 0; base < n; base += 16) {
     const int jj = base + lane;
@@ -784,10 +769,9 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
       keep = dx * dx + dy * dy + dz * dz <= cutsq_in;
       if (MIXED) {
         const int cj = scell[j];
-        const int jx = cj % g.nc[0], jy = (cj / g.nc[0]) % g.nc[1], jz = cj / (g.nc[0] * g.nc[1]);
-        const int ox = jx + tx * g.nc[0] - icp[0];
-        const int oy = jy + ty * g.nc[1] - icp[1];
-        const int oz = jz + tz * g.nc[2] - icp[2];
+        const int ox = cell_ix(cj) + tx * g.nc[0] - icx;
+        const int oy = cell_iy(cj) + ty * g.nc[1] - icy;
+        const int oz = cell_iz(cj) + tz * g.nc[2] - icz;
         const unsigned dcode = (unsigned)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
         eout = (unsigned)j | (dcode << MIX_SHIFT);
       } else {
@@ -797,14 +781,10 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
     const unsigned long long m = __ballot(keep);
     const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
     const int rank = __popc(gm & ((1u << lane) - 1u));
-    if (keep && count + rank < icap) out[count + rank] = eout;
+    if (keep) out[count + rank] = eout;
     count += __popc(gm);
   }
-  if (active && lane == 0) {
-    innum[r] = count < icap ? count : icap;
-    atomicMax(&flags->maxcount, count);
-    if (count > icap) atomicOr(&flags->overflow, 1);
-  }
+  if (lane == 0) innum[r] = count;
 }
 
 void build_inner(Ctx& c) {
@@ -814,26 +794,18 @@ void build_inner(Ctx& c) {
   const double rin = c.cutforce + c.inner_skin;
   c.inner_cutsq = rin * rin;
   Grid g = grid_of(c);
-  if (c.icap == 0) c.icap = 64;
+  c.icap = c.cap;
   c.innum.reserve(c.P);
-  for (int attempt = 0; attempt < 3; attempt++) {
-    c.ilist.reserve((size_t)c.icap * c.P);
-    CAC_HIP(hipMemsetAsync(&c.d_flags.get()->overflow, 0, sizeof(int) * 2, c.stream));
-    const int blocks = grid_for(16L * c.P, 256);
-    const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
-    unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());
-    if (mixed)
-      k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.nnum.get(), nl,
-                                                   c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il, c.icap,
-                                                   c.d_flags.get());
-    else
-      k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.nnum.get(), nl,
-                                                    c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il, c.icap,
-                                                    c.d_flags.get());
-    read_flags(c);
-    if (!c.h_flags->overflow) break;
-    c.icap = ((c.h_flags->maxcount + 15) / 16) * 16 + 16;
-  }
+  c.ilist.reserve((size_t)c.icap * c.P);
+  const int blocks = grid_for(16L * c.P, 256);
+  const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
+  unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());
+  if (mixed)
+    k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.nnum.get(), nl, c.P,
+                                                 c.cap, g, c.inner_cutsq, c.innum.get(), il);
+  else
+    k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.nnum.get(), nl, c.P,
+                                                  c.cap, g, c.inner_cutsq, c.innum.get(), il);
   CAC_HIP(hipMemcpyAsync(c.xhold_in.get(), c.x.get(), sizeof(double) * 3 * c.P, hipMemcpyDeviceToDevice, c.stream));
   c.inner_mode = c.precision;
   c.inner_builds++;

@@ -96,73 +96,51 @@ __device__ __forceinline__ T row_sum(T v) {
 }
 
 // ------------------------------------------------------------------ site refresh
-// forward_comm equivalent: every site position from its source point or from
-// the shape-function interpolation of its element's nodes, plus its wrap
-// shift.  Points: one thread each.  Interpolated atoms: one wavefront per
-// element, which loads the element's 8 nodes once and walks its live sites.
+// forward_comm equivalent: every live site's position from its source point,
+// or from the shape-function interpolation of its element's nodes, plus its
+// wrap shift.  One thread per site in site order, so the record writes are
+// coalesced; the node and shape-function reads of neighbouring sites (same
+// cell, mostly the same element) hit the cache.  The mixed path keeps a FP32
+// copy relative to the corner of the site's neighbor cell; its FP64 records
+// are written only when a list is built or pruned.
 
 struct Frame {
   double L[3], lo[3], cs[3];
-  int nc0, nc1;
 };
 
-__device__ __forceinline__ void write_site(double4* site, float4* site32, const int* scell, const Frame& fr, int k,
-                                           const double* y) {
-  double* out = reinterpret_cast<double*>(site + k);
-  out[0] = y[0];
-  out[1] = y[1];
-  out[2] = y[2];
-  if (site32) {
-    const int cc = scell[k];
-    const int ix = cc % fr.nc0, iy = (cc / fr.nc0) % fr.nc1, iz = cc / (fr.nc0 * fr.nc1);
-    float* o32 = reinterpret_cast<float*>(site32 + k);
-    o32[0] = (float)(y[0] - (fr.lo[0] + ix * fr.cs[0]));
-    o32[1] = (float)(y[1] - (fr.lo[1] + iy * fr.cs[1]));
-    o32[2] = (float)(y[2] - (fr.lo[2] + iz * fr.cs[2]));
+template <bool W64, bool W32>
+__global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, const double* __restrict__ shape,
+                                                 const int* __restrict__ sdesc, const int* __restrict__ simg,
+                                                 const int* __restrict__ scell, int S, int A, int P, int nucell,
+                                                 Frame fr, double4* __restrict__ site, float4* __restrict__ site32) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S) return;
+  const int d = sdesc[k];
+  double y[3];
+  if (d >= 0) {
+    for (int c = 0; c < 3; c++) y[c] = x[(size_t)c * P + d];
+  } else {
+    const int q = -d - 1;
+    const int e = q / nucell, u = q - e * nucell;
+    const double* N = shape + (size_t)u * NPE;
+    const size_t base = (size_t)A + (size_t)NPE * e;
+    for (int c = 0; c < 3; c++) {
+      const double* nx = x + (size_t)c * P + base;
+      double s = 0.0;
+      for (int n = 0; n < NPE; n++) s += N[n] * nx[n];
+      y[c] = s;
+    }
   }
-}
-
-__global__ void k_refresh_points(const double* __restrict__ x, const int* __restrict__ point_site,
-                                 const int* __restrict__ simg, const int* __restrict__ scell, int P, Frame fr,
-                                 double4* __restrict__ site, float4* __restrict__ site32) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  const int k = point_site[p];
   double sh[3];
   img_shift((unsigned)simg[k], fr.L, sh[0], sh[1], sh[2]);
-  double y[3];
-  for (int c = 0; c < 3; c++) y[c] = x[(size_t)c * P + p] + sh[c];
-  write_site(site, site32, scell, fr, k, y);
-}
-
-__global__ __launch_bounds__(64) void k_refresh_elements(const double* __restrict__ x,
-                                                         const double* __restrict__ shape,
-                                                         const int* __restrict__ estart,
-                                                         const int* __restrict__ eslot, const int* __restrict__ sdesc,
-                                                         const int* __restrict__ simg, const int* __restrict__ scell,
-                                                         int A, int P, int nucell, Frame fr, double4* __restrict__ site,
-                                                         float4* __restrict__ site32) {
-  const int e = blockIdx.x;
-  __shared__ double nx[3][NPE];
-  if (threadIdx.x < 3 * NPE) {
-    const int c = threadIdx.x / NPE, n = threadIdx.x % NPE;
-    nx[c][n] = x[(size_t)c * P + A + (size_t)NPE * e + n];
-  }
-  __syncthreads();
-  const int i1 = estart[e + 1];
-  for (int i = estart[e] + threadIdx.x; i < i1; i += blockDim.x) {
-    const int k = eslot[i];
-    const int u = (-sdesc[k] - 1) - e * nucell;
-    const double* N = shape + (size_t)u * NPE;
-    double sh[3];
-    img_shift((unsigned)simg[k], fr.L, sh[0], sh[1], sh[2]);
-    double y[3];
-    for (int c = 0; c < 3; c++) {
-      double s = 0.0;
-      for (int n = 0; n < NPE; n++) s += N[n] * nx[c][n];
-      y[c] = s + sh[c];
-    }
-    write_site(site, site32, scell, fr, k, y);
+  for (int c = 0; c < 3; c++) y[c] = y[c] + sh[c];
+  if (W64) site[k] = make_double4(y[0], y[1], y[2], 0.0);
+  if (W32) {
+    const int cc = scell[k];
+    site32[k] = make_float4((float)(y[0] - (fr.lo[0] + cell_ix(cc) * fr.cs[0])),
+                            (float)(y[1] - (fr.lo[1] + cell_iy(cc) * fr.cs[1])),
+                            (float)(y[2] - (fr.lo[2] + cell_iz(cc) * fr.cs[2])), 0.0f);
   }
 }
 
@@ -173,24 +151,28 @@ static Frame frame_of(const Ctx& c) {
     fr.lo[d] = c.lo[d];
     fr.cs[d] = c.L[d] / c.nc[d];
   }
-  fr.nc0 = c.nc[0];
-  fr.nc1 = c.nc[1];
   return fr;
 }
 
-void site_refresh(Ctx& c) {
+void site_refresh(Ctx& c, bool full) {
   ScopedTimer tm(c, "site_refresh");
   c.x_dirty = false;
   const bool mixed = c.precision == 1;
   if (mixed) c.site32.reserve(c.S);
   const Frame fr = frame_of(c);
-  float4* s32 = mixed ? c.site32.get() : nullptr;
-  k_refresh_points<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.point_site.get(), c.simg.get(),
-                                                             c.scell.get(), c.P, fr, c.site.get(), s32);
-  if (c.E && c.nvirt > 0)
-    k_refresh_elements<<<c.E, 64, 0, c.stream>>>(c.x.get(), c.shape.get(), c.estart.get(), c.eslot.get(),
-                                                 c.sdesc.get(), c.simg.get(), c.scell.get(), c.A, c.P, c.nucell, fr,
-                                                 c.site.get(), s32);
+  const int nuc = c.nucell > 0 ? c.nucell : 1;
+  const int grid = grid_for(c.S, 256);
+  if (!mixed)
+    k_refresh<true, false><<<grid, 256, 0, c.stream>>>(c.x.get(), c.shape.get(), c.sdesc.get(), c.simg.get(),
+                                                       c.scell.get(), c.S, c.A, c.P, nuc, fr, c.site.get(), nullptr);
+  else if (full)
+    k_refresh<true, true><<<grid, 256, 0, c.stream>>>(c.x.get(), c.shape.get(), c.sdesc.get(), c.simg.get(),
+                                                      c.scell.get(), c.S, c.A, c.P, nuc, fr, c.site.get(),
+                                                      c.site32.get());
+  else
+    k_refresh<false, true><<<grid, 256, 0, c.stream>>>(c.x.get(), c.shape.get(), c.sdesc.get(), c.simg.get(),
+                                                       c.scell.get(), c.S, c.A, c.P, nuc, fr, nullptr,
+                                                       c.site32.get());
 }
 
 // ------------------------------------------------------------------ spline helpers
@@ -329,38 +311,35 @@ __device__ __forceinline__ double fprime(const double* __restrict__ frho, double
   return (c[0] * p + c[1]) * p + c[2];
 }
 
-__global__ void k_embed_points(const double* __restrict__ rho, const int* __restrict__ point_site, int P,
-                               const double* __restrict__ frho, double rdrho, int nrho, double4* __restrict__ site,
-                               float4* __restrict__ site32) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  const int k = point_site[p];
-  const double fp = fprime(frho, rho[p], rdrho, nrho);
-  reinterpret_cast<double*>(site + k)[3] = fp;
-  if (site32) reinterpret_cast<float*>(site32 + k)[3] = (float)fp;
-}
-
-__global__ __launch_bounds__(64) void k_embed_elements(const double* __restrict__ rho,
-                                                       const double* __restrict__ shape,
-                                                       const int* __restrict__ estart, const int* __restrict__ eslot,
-                                                       const int* __restrict__ sdesc, int A, int nucell,
-                                                       const double* __restrict__ frho, double rdrho, int nrho,
-                                                       double4* __restrict__ site, float4* __restrict__ site32) {
-  const int e = blockIdx.x;
-  __shared__ double nr[NPE];
-  if (threadIdx.x < NPE) nr[threadIdx.x] = rho[(size_t)A + (size_t)NPE * e + threadIdx.x];
-  __syncthreads();
-  const int i1 = estart[e + 1];
-  for (int i = estart[e] + threadIdx.x; i < i1; i += blockDim.x) {
-    const int k = eslot[i];
-    const int u = (-sdesc[k] - 1) - e * nucell;
+// F'(rho) of every live site (compute_fp, pair_eam.cpp:1160-1169): a point's
+// own density, or the shape-function interpolation of its element's nodal
+// densities for an interpolated atom (pair_eam.cpp:700-760); one thread per
+// site in site order.  Only the F' word of each record is written.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, const double* __restrict__ shape,
+                                               const int* __restrict__ sdesc, int S, int A, int nucell,
+                                               const double* __restrict__ frho, double rdrho, int nrho,
+                                               double4* __restrict__ site, float4* __restrict__ site32) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S) return;
+  const int d = sdesc[k];
+  double rj;
+  if (d >= 0) {
+    rj = rho[d];
+  } else {
+    const int q = -d - 1;
+    const int e = q / nucell, u = q - e * nucell;
     const double* N = shape + (size_t)u * NPE;
-    double rj = 0.0;
+    const double* nr = rho + (size_t)A + (size_t)NPE * e;
+    rj = 0.0;
     for (int n = 0; n < NPE; n++) rj += N[n] * nr[n];
-    const double fp = fprime(frho, rj, rdrho, nrho);
-    reinterpret_cast<double*>(site + k)[3] = fp;
-    if (site32) reinterpret_cast<float*>(site32 + k)[3] = (float)fp;
   }
+  const double fp = fprime(frho, rj, rdrho, nrho);
+  if (MIXED)
+    reinterpret_cast<float*>(site32 + k)[3] = (float)fp;
+  else
+    reinterpret_cast<double*>(site + k)[3] = fp;
 }
 
 // ------------------------------------------------------------------ EAM force (pair_eam.cpp:853-1027)
@@ -504,7 +483,7 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
       if (EV) {
         // embedding energy F(rho) at atoms (weight 1) and nodes (weight nodal_weight)
         const double rho = q.rho[pnt];
-        const double fpi = a.site[k].w;
+        const double fpi = fprime(q.frho, rho, q.rdrho, q.nrho);
         double p;
         const int m = knot_rho(rho, q.rdrho, q.nrho, p);
         const double* c = q.frho + (size_t)m * 7;
@@ -705,13 +684,15 @@ static void launch_pair(Ctx& c, int eflag, int vflag) {
     }
     {
       ScopedTimer tm(c, "eam_embed");
-      float4* s32 = mixed ? c.site32.get() : nullptr;
-      k_embed_points<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.rho.get(), c.point_site.get(), c.P,
-                                                               c.frho_spl.get(), c.rdrho, c.nrho, c.site.get(), s32);
-      if (c.E && c.nvirt > 0)
-        k_embed_elements<<<c.E, 64, 0, c.stream>>>(c.rho.get(), c.shape.get(), c.estart.get(), c.eslot.get(),
-                                                   c.sdesc.get(), c.A, c.nucell, c.frho_spl.get(), c.rdrho, c.nrho,
-                                                   c.site.get(), s32);
+      const int nuc = c.nucell > 0 ? c.nucell : 1;
+      if (mixed)
+        k_embed<true><<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.rho.get(), c.shape.get(), c.sdesc.get(), c.S, c.A,
+                                                                 nuc, c.frho_spl.get(), c.rdrho, c.nrho, nullptr,
+                                                                 c.site32.get());
+      else
+        k_embed<false><<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.rho.get(), c.shape.get(), c.sdesc.get(), c.S, c.A,
+                                                                  nuc, c.frho_spl.get(), c.rdrho, c.nrho, c.site.get(),
+                                                                  nullptr);
     }
     EamForceArgs q;
     q.tab = tab;
@@ -761,8 +742,8 @@ static void launch_pair(Ctx& c, int eflag, int vflag) {
 void pair_compute(Ctx& c, int eflag, int vflag) {
   const bool ev = eflag || vflag;
   if (ev) c.erow.reserve((size_t)7 * c.P);
-  if (c.inner_mode != c.precision) {  // precision changed since the last prune
-    site_refresh(c);
+  if (c.inner_mode != c.precision) {  // precision or delta changed since the last prune
+    site_refresh(c, true);
     build_inner(c);
   }
   switch (c.tpr) {
