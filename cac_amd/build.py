@@ -23,6 +23,9 @@ CXXFLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 ops stay off: they are not needed and some forms misbehave on this part
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # contract a * b + c only inside one expression, so results do not depend
+    # on unrolling or on the launch shape
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
     f"-I{os.path.join(ROOT, 'include')}",
 ]

@@ -195,6 +195,8 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     c.units = units;
     if (const char* e = std::getenv("CAC_HIP_TPR")) c.tpr = std::atoi(e);
     if (const char* e = std::getenv("CAC_HIP_GRID")) c.grid_cap = std::max(8, std::atoi(e));
+    if (const char* e = std::getenv("CAC_HIP_PAIR_SHAPE")) c.pair_shape = std::atoi(e);
+    CAC_HIP(hipDeviceGetAttribute(&c.ncu, hipDeviceAttributeMultiprocessorCount, device));
     if (units == CAC_HIP_UNITS_LJ) {
       c.boltz = 1.0;
       c.mvv2e = 1.0;

@@ -43,6 +43,34 @@ __host__ __device__ inline int cell_pack(int ix, int iy, int iz) {
 __host__ __device__ inline int cell_ix(int p) { return p & ((1 << CELL_BITS_X) - 1); }
 __host__ __device__ inline int cell_iy(int p) { return (p >> CELL_BITS_X) & ((1 << CELL_BITS_Y) - 1); }
 __host__ __device__ inline int cell_iz(int p) { return (p >> (CELL_BITS_X + CELL_BITS_Y)) & ((1 << CELL_BITS_Z) - 1); }
+
+// Trilinear Hex shape functions of unit cell u (i-major: u = (i ny + j) nz + k)
+// at natural coordinates (h_x i - 1, h_y j - 1, h_z k - 1), h = 2 / (n - 1),
+// formed with the same operations, in the same order and unfused, as the
+// element set-up forms its shape table (element.cpp:2989-2997), so the values
+// are bitwise those of the table.
+struct UcellGrid {
+  double h[3];
+  int ny, nz;
+};
+
+__device__ __forceinline__ void hex_shape_at(const UcellGrid& ug, int u, double* N) {
+#pragma clang fp contract(off)
+  const int nyz = ug.ny * ug.nz;
+  const int i = u / nyz;
+  const int r = u - i * nyz;
+  const int j = r / ug.nz;
+  const int k = r - j * ug.nz;
+  const double px = ug.h[0] * i - 1.0, py = ug.h[1] * j - 1.0, pz = ug.h[2] * k - 1.0;
+  N[0] = 0.125 * (1 - px) * (1 - py) * (1 - pz);
+  N[1] = 0.125 * (1 + px) * (1 - py) * (1 - pz);
+  N[2] = 0.125 * (1 + px) * (1 + py) * (1 - pz);
+  N[3] = 0.125 * (1 - px) * (1 + py) * (1 - pz);
+  N[4] = 0.125 * (1 - px) * (1 - py) * (1 + pz);
+  N[5] = 0.125 * (1 + px) * (1 - py) * (1 + pz);
+  N[6] = 0.125 * (1 + px) * (1 + py) * (1 + pz);
+  N[7] = 0.125 * (1 - px) * (1 + py) * (1 + pz);
+}
 constexpr int MAXT = 8;
 
 struct HipError {
@@ -185,8 +213,10 @@ struct Ctx {
   DBuf<float> tab32;   // per knot: rhor c3..c6, z2r c3..c6 (FP32, mixed path)
   DBuf<double> tab64;  // the same in FP64
   int precision = 0;     // 0 FP64, 1 mixed
-  int tpr = 16;          // lanes per neighbor row in the pair kernels
-  int grid_cap = 2048;   // max workgroups of the persistent pair kernels
+  int tpr = 8;           // lanes per neighbor row in the pair kernels
+  int grid_cap = 4096;   // max workgroups of the persistent pair kernels
+  int pair_shape = 0;    // EAM pass shape: 0/1/2/3 = (512|1024 threads) x (4|8 entries in flight)
+  int ncu = 256;         // compute units of the device
   int nodef_style = 0;   // 0 consistent, 1 lumped, 2 node/only
 
   // energy / virial

@@ -741,6 +741,7 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
                                                const unsigned* __restrict__ nlist, int R, int cap, Grid g,
                                                double cutsq_in, int* __restrict__ innum,
                                                unsigned* __restrict__ ilist) {
+  constexpr int PU = 4;  // 16-entry groups in flight per lane group
   const int lane = threadIdx.x & 15;
   const int gbase = (threadIdx.x & 63) & ~15;
   const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
@@ -753,36 +754,43 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
   const unsigned* row = nlist + (size_t)r * cap;
   unsigned* out = ilist + (size_t)r * cap;
   int count = 0;
-  for (int base = 0; base < n; base += 16) {
-    const int jj = base + lane;
-    bool keep = false;
-    unsigned eout = 0u;
-    if (jj < n) {
-      const unsigned ent = row[jj];
-      const int j = (int)(ent & SITE_MASK);
-      const unsigned t = ent >> IMG_SHIFT;
-      const int tx = (int)(t % 3u) - 1, ty = (int)((t / 3u) % 3u) - 1, tz = (int)(t / 9u) - 1;
-      const double4 sj = site[j];
-      const double dx = pk.x - (sj.x + (double)tx * g.L[0]);
-      const double dy = pk.y - (sj.y + (double)ty * g.L[1]);
-      const double dz = pk.z - (sj.z + (double)tz * g.L[2]);
-      keep = dx * dx + dy * dy + dz * dz <= cutsq_in;
-      if (MIXED) {
-        const int cj = scell[j];
-        const int ox = cell_ix(cj) + tx * g.nc[0] - icx;
-        const int oy = cell_iy(cj) + ty * g.nc[1] - icy;
-        const int oz = cell_iz(cj) + tz * g.nc[2] - icz;
-        const unsigned dcode = (unsigned)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
-        eout = (unsigned)j | (dcode << MIX_SHIFT);
-      } else {
-        eout = ent;
-      }
+  for (int base = 0; base < n; base += 16 * PU) {
+    unsigned ent[PU];
+    double4 sj[PU];
+    int cj[PU];
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+      const int jj = base + 16 * u + lane;
+      ent[u] = jj < n ? row[jj] : (unsigned)k | (13u << IMG_SHIFT);
     }
-    const unsigned long long m = __ballot(keep);
-    const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
-    const int rank = __popc(gm & ((1u << lane) - 1u));
-    if (keep) out[count + rank] = eout;
-    count += __popc(gm);
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+      sj[u] = site[ent[u] & SITE_MASK];
+      if (MIXED) cj[u] = scell[ent[u] & SITE_MASK];
+    }
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+      const int jj = base + 16 * u + lane;
+      const unsigned t = ent[u] >> IMG_SHIFT;
+      const int tx = (int)(t % 3u) - 1, ty = (int)((t / 3u) % 3u) - 1, tz = (int)(t / 9u) - 1;
+      const double dx = pk.x - (sj[u].x + (double)tx * g.L[0]);
+      const double dy = pk.y - (sj[u].y + (double)ty * g.L[1]);
+      const double dz = pk.z - (sj[u].z + (double)tz * g.L[2]);
+      const bool keep = jj < n && dx * dx + dy * dy + dz * dz <= cutsq_in;
+      unsigned eout = ent[u];
+      if (MIXED) {
+        const int ox = cell_ix(cj[u]) + tx * g.nc[0] - icx;
+        const int oy = cell_iy(cj[u]) + ty * g.nc[1] - icy;
+        const int oz = cell_iz(cj[u]) + tz * g.nc[2] - icz;
+        const unsigned dcode = (unsigned)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
+        eout = (ent[u] & SITE_MASK) | (dcode << MIX_SHIFT);
+      }
+      const unsigned long long m = __ballot(keep);
+      const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
+      const int rank = __popc(gm & ((1u << lane) - 1u));
+      if (keep) out[count + rank] = eout;
+      count += __popc(gm);
+    }
   }
   if (lane == 0) innum[r] = count;
 }

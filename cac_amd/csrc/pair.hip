@@ -109,7 +109,7 @@ struct Frame {
 };
 
 template <bool W64, bool W32>
-__global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, const double* __restrict__ shape,
+__global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, UcellGrid ug,
                                                  const int* __restrict__ sdesc, const int* __restrict__ simg,
                                                  const int* __restrict__ scell, int S, int A, int P, int nucell,
                                                  Frame fr, double4* __restrict__ site, float4* __restrict__ site32) {
@@ -123,7 +123,8 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, c
   } else {
     const int q = -d - 1;
     const int e = q / nucell, u = q - e * nucell;
-    const double* N = shape + (size_t)u * NPE;
+    double N[NPE];
+    hex_shape_at(ug, u, N);
     const size_t base = (size_t)A + (size_t)NPE * e;
     for (int c = 0; c < 3; c++) {
       const double* nx = x + (size_t)c * P + base;
@@ -144,6 +145,14 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, c
   }
 }
 
+static UcellGrid ucell_grid(const Ctx& c) {
+  UcellGrid ug;
+  for (int d = 0; d < 3; d++) ug.h[d] = c.ncell[d] > 1 ? 2.0 / (c.ncell[d] - 1) : 0.0;
+  ug.ny = c.ncell[1] > 0 ? c.ncell[1] : 1;
+  ug.nz = c.ncell[2] > 0 ? c.ncell[2] : 1;
+  return ug;
+}
+
 static Frame frame_of(const Ctx& c) {
   Frame fr;
   for (int d = 0; d < 3; d++) {
@@ -160,17 +169,18 @@ void site_refresh(Ctx& c, bool full) {
   const bool mixed = c.precision == 1;
   if (mixed) c.site32.reserve(c.S);
   const Frame fr = frame_of(c);
+  const UcellGrid ug = ucell_grid(c);
   const int nuc = c.nucell > 0 ? c.nucell : 1;
   const int grid = grid_for(c.S, 256);
   if (!mixed)
-    k_refresh<true, false><<<grid, 256, 0, c.stream>>>(c.x.get(), c.shape.get(), c.sdesc.get(), c.simg.get(),
+    k_refresh<true, false><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(),
                                                        c.scell.get(), c.S, c.A, c.P, nuc, fr, c.site.get(), nullptr);
   else if (full)
-    k_refresh<true, true><<<grid, 256, 0, c.stream>>>(c.x.get(), c.shape.get(), c.sdesc.get(), c.simg.get(),
+    k_refresh<true, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(),
                                                       c.scell.get(), c.S, c.A, c.P, nuc, fr, c.site.get(),
                                                       c.site32.get());
   else
-    k_refresh<false, true><<<grid, 256, 0, c.stream>>>(c.x.get(), c.shape.get(), c.sdesc.get(), c.simg.get(),
+    k_refresh<false, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(),
                                                        c.scell.get(), c.S, c.A, c.P, nuc, fr, nullptr,
                                                        c.site32.get());
 }
@@ -209,6 +219,12 @@ __device__ __forceinline__ void stage_table(V* lds, const V* __restrict__ g, int
   __syncthreads();
 }
 
+template <typename V>
+__device__ __forceinline__ void stage_table_strided(V* lds, const V* __restrict__ g, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[2 * i];
+  __syncthreads();
+}
+
 // ------------------------------------------------------------------ row walker
 // Lane l of a row's TPR lanes visits entries l, l + TPR, ...; U entries at a
 // time: U list loads, then U site gathers, then the U pair terms, so U
@@ -237,16 +253,17 @@ __device__ __forceinline__ void walk_row(const unsigned* __restrict__ row, int n
 
 // ------------------------------------------------------------------ EAM density (pair_eam.cpp:721-813)
 
-template <int TPR, bool MIXED, int NT>
+// the density pass stages only the rho(r) half of the table (knot m at m)
+template <int TPR, bool MIXED, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __restrict__ tab, double rdr, int nr,
                                                     double* __restrict__ rho_out) {
   extern __shared__ float4 lds_raw[];
   float4* lds32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
   if (MIXED)
-    stage_table(lds32, static_cast<const float4*>(tab), 2 * (nr + 1));
+    stage_table_strided(lds32, static_cast<const float4*>(tab), nr + 1);
   else
-    stage_table(lds64, static_cast<const double4*>(tab), 2 * (nr + 1));
+    stage_table_strided(lds64, static_cast<const double4*>(tab), nr + 1);
   const int lane = threadIdx.x % TPR;
   const int rsub = threadIdx.x / TPR;
   const Sched s = sched(a.nchunks);
@@ -261,7 +278,7 @@ __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __re
       const unsigned* row = a.nlist + (size_t)r * a.cap;
       if (MIXED) {
         const float4 pi = a.site32[k];
-        walk_row<TPR, UNROLL, float4>(
+        walk_row<TPR, U, float4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
             [&](unsigned ent, const float4& sj) {
               float dx, dy, dz;
@@ -270,13 +287,13 @@ __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __re
               if (rsq < cut32) {
                 float p;
                 const int m = knot_r<float>(__fsqrt_rn(rsq), rdr32, nr, p);
-                const float4 c = lds32[2 * m];
+                const float4 c = lds32[m];
                 acc += (double)(((c.x * p + c.y) * p + c.z) * p + c.w);
               }
             });
       } else {
         const double4 pi = a.site[k];
-        walk_row<TPR, UNROLL, double4>(
+        walk_row<TPR, U, double4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
             [&](unsigned ent, const double4& sj) {
               double shx, shy, shz;
@@ -288,7 +305,7 @@ __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __re
               if (rsq < a.cutsq) {
                 double p;
                 const int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
-                const double4 c = lds64[2 * m];
+                const double4 c = lds64[m];
                 acc += ((c.x * p + c.y) * p + c.z) * p + c.w;
               }
             });
@@ -316,7 +333,7 @@ __device__ __forceinline__ double fprime(const double* __restrict__ frho, double
 // densities for an interpolated atom (pair_eam.cpp:700-760); one thread per
 // site in site order.  Only the F' word of each record is written.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, const double* __restrict__ shape,
+__global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, UcellGrid ug,
                                                const int* __restrict__ sdesc, int S, int A, int nucell,
                                                const double* __restrict__ frho, double rdrho, int nrho,
                                                double4* __restrict__ site, float4* __restrict__ site32) {
@@ -330,7 +347,8 @@ __global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, c
   } else {
     const int q = -d - 1;
     const int e = q / nucell, u = q - e * nucell;
-    const double* N = shape + (size_t)u * NPE;
+    double N[NPE];
+    hex_shape_at(ug, u, N);
     const double* nr = rho + (size_t)A + (size_t)NPE * e;
     rj = 0.0;
     for (int n = 0; n < NPE; n++) rj += N[n] * nr[n];
@@ -353,7 +371,7 @@ struct EamForceArgs {
   double escale_node, vscale_node;
 };
 
-template <int TPR, bool MIXED, bool EV, int NT>
+template <int TPR, bool MIXED, bool EV, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, double* __restrict__ f,
                                                   double* __restrict__ gf, int P, int E8,
                                                   double* __restrict__ erow) {
@@ -380,7 +398,7 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
       if (MIXED) {
         const float4 pi = a.site32[k];
         const float cut32 = (float)a.cutsq;
-        walk_row<TPR, UNROLL, float4>(
+        walk_row<TPR, U, float4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
             [&](unsigned ent, const float4& sj) {
               float dx, dy, dz;
@@ -416,7 +434,7 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
             });
       } else {
         const double4 pi = a.site[k];
-        walk_row<TPR, UNROLL, double4>(
+        walk_row<TPR, U, double4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
             [&](unsigned ent, const double4& sj) {
               double shx, shy, shz;
@@ -659,6 +677,65 @@ static void allow_lds(K kernel, size_t bytes) {
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+// Workgroups of a persistent kernel: as many as can be resident at once
+// (registers, LDS and waves per CU as the runtime reports them), so that no
+// workgroup waits for another to finish before it starts its share.
+template <typename K>
+static int resident_grid(Ctx& c, K kernel, int nt, size_t lds, long nchunks) {
+  allow_lds(kernel, lds);
+  int per_cu = 0;
+  CAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), nt, lds));
+  if (per_cu < 1) per_cu = 1;
+  return pair_grid(nchunks, std::min<long>((long)per_cu * c.ncu, c.grid_cap));
+}
+
+template <int TPR, bool MIXED, int NT, int U>
+static void run_eam(Ctx& c, int eflag, int vflag) {
+  const bool ev = eflag || vflag;
+  const int E8 = NPE * c.E;
+  double* gf = c.gf.get();
+  double* er = ev ? c.erow.get() : nullptr;
+  const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
+  const size_t lds_rho = knot * (c.nr + 1), lds_force = 2 * knot * (c.nr + 1);
+  PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
+  const void* tab = MIXED ? (const void*)c.tab32.get() : (const void*)c.tab64.get();
+  {
+    ScopedTimer tm(c, "eam_density");
+    auto kd = k_eam_density<TPR, MIXED, NT, U>;
+    const int grid = resident_grid(c, kd, NT, lds_rho, a.nchunks);
+    kd<<<grid, NT, lds_rho, c.stream>>>(a, tab, c.rdr, c.nr, c.rho.get());
+  }
+  {
+    ScopedTimer tm(c, "eam_embed");
+    const int nuc = c.nucell > 0 ? c.nucell : 1;
+    k_embed<MIXED><<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.rho.get(), ucell_grid(c), c.sdesc.get(), c.S, c.A, nuc,
+                                                             c.frho_spl.get(), c.rdrho, c.nrho,
+                                                             MIXED ? nullptr : c.site.get(),
+                                                             MIXED ? c.site32.get() : nullptr);
+  }
+  EamForceArgs q;
+  q.tab = tab;
+  q.frho = c.frho_spl.get();
+  q.rho = c.rho.get();
+  q.rdr = c.rdr;
+  q.rdrho = c.rdrho;
+  q.rhomax = c.rhomax;
+  q.nr = c.nr;
+  q.nrho = c.nrho;
+  q.escale_node = c.nodal_weight / 2.0;
+  q.vscale_node = c.nodal_weight / 2.0;
+  ScopedTimer tm(c, "eam_force");
+  if (ev) {
+    auto kf = k_eam_force<TPR, MIXED, true, NT, U>;
+    const int grid = resident_grid(c, kf, NT, lds_force, a.nchunks);
+    kf<<<grid, NT, lds_force, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+  } else {
+    auto kf = k_eam_force<TPR, MIXED, false, NT, U>;
+    const int grid = resident_grid(c, kf, NT, lds_force, a.nchunks);
+    kf<<<grid, NT, lds_force, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+  }
+}
+
 template <int TPR>
 static void launch_pair(Ctx& c, int eflag, int vflag) {
   const bool ev = eflag || vflag;
@@ -667,75 +744,29 @@ static void launch_pair(Ctx& c, int eflag, int vflag) {
   double* gf = c.gf.get();
   double* er = ev ? c.erow.get() : nullptr;
   if (c.pairstyle == 2) {
-    constexpr int NT32 = 512, NT64 = 1024;  // 64 KB table: two workgroups per CU; 128 KB: one
-    const int nt = mixed ? NT32 : NT64;
-    PairArgs a = pair_args(c, c.cutforcesq, nt / TPR);
-    const int grid = pair_grid(a.nchunks, mixed ? 512 : 256);
-    const size_t lds = (mixed ? sizeof(float4) : sizeof(double4)) * 2 * (c.nr + 1);
-    const void* tab = mixed ? (const void*)c.tab32.get() : (const void*)c.tab64.get();
-    {
-      ScopedTimer tm(c, "eam_density");
-      if (mixed) {
-        k_eam_density<TPR, true, NT32><<<grid, NT32, lds, c.stream>>>(a, tab, c.rdr, c.nr, c.rho.get());
-      } else {
-        allow_lds(k_eam_density<TPR, false, NT64>, lds);
-        k_eam_density<TPR, false, NT64><<<grid, NT64, lds, c.stream>>>(a, tab, c.rdr, c.nr, c.rho.get());
-      }
-    }
-    {
-      ScopedTimer tm(c, "eam_embed");
-      const int nuc = c.nucell > 0 ? c.nucell : 1;
-      if (mixed)
-        k_embed<true><<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.rho.get(), c.shape.get(), c.sdesc.get(), c.S, c.A,
-                                                                 nuc, c.frho_spl.get(), c.rdrho, c.nrho, nullptr,
-                                                                 c.site32.get());
-      else
-        k_embed<false><<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.rho.get(), c.shape.get(), c.sdesc.get(), c.S, c.A,
-                                                                  nuc, c.frho_spl.get(), c.rdrho, c.nrho, c.site.get(),
-                                                                  nullptr);
-    }
-    EamForceArgs q;
-    q.tab = tab;
-    q.frho = c.frho_spl.get();
-    q.rho = c.rho.get();
-    q.rdr = c.rdr;
-    q.rdrho = c.rdrho;
-    q.rhomax = c.rhomax;
-    q.nr = c.nr;
-    q.nrho = c.nrho;
-    q.escale_node = c.nodal_weight / 2.0;
-    q.vscale_node = c.nodal_weight / 2.0;
-    ScopedTimer tm(c, "eam_force");
+    // workgroup size and entries in flight per lane of the EAM passes
     if (mixed) {
-      if (ev)
-        k_eam_force<TPR, true, true, NT32><<<grid, NT32, lds, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
-      else
-        k_eam_force<TPR, true, false, NT32><<<grid, NT32, lds, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+      switch (c.pair_shape) {
+        case 1: run_eam<TPR, true, 1024, 4>(c, eflag, vflag); break;
+        case 2: run_eam<TPR, true, 512, 8>(c, eflag, vflag); break;
+        case 3: run_eam<TPR, true, 1024, 8>(c, eflag, vflag); break;
+        default: run_eam<TPR, true, 512, 4>(c, eflag, vflag); break;
+      }
     } else {
-      if (ev) {
-        allow_lds(k_eam_force<TPR, false, true, NT64>, lds);
-        k_eam_force<TPR, false, true, NT64><<<grid, NT64, lds, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
-      } else {
-        allow_lds(k_eam_force<TPR, false, false, NT64>, lds);
-        k_eam_force<TPR, false, false, NT64><<<grid, NT64, lds, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+      switch (c.pair_shape) {
+        case 2:
+        case 3: run_eam<TPR, false, 1024, 8>(c, eflag, vflag); break;
+        default: run_eam<TPR, false, 1024, 4>(c, eflag, vflag); break;
       }
     }
   } else {
     PairArgs a = pair_args(c, c.ljcutsq, PB / TPR);
-    const int grid = pair_grid(a.nchunks, c.grid_cap);
     LjArgs q{c.lj1, c.lj2, c.lj3, c.lj4, c.nodal_weight / 2.0, c.nodal_weight / 2.0};
     ScopedTimer tm(c, "lj_force");
-    if (mixed) {
-      if (ev)
-        k_lj_force<TPR, true, true><<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
-      else
-        k_lj_force<TPR, true, false><<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
-    } else {
-      if (ev)
-        k_lj_force<TPR, false, true><<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
-      else
-        k_lj_force<TPR, false, false><<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
-    }
+    auto kl = mixed ? (ev ? k_lj_force<TPR, true, true> : k_lj_force<TPR, true, false>)
+                    : (ev ? k_lj_force<TPR, false, true> : k_lj_force<TPR, false, false>);
+    const int grid = resident_grid(c, kl, PB, 0, a.nchunks);
+    kl<<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
   }
 }
 
