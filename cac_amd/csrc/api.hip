@@ -457,15 +457,20 @@ int cac_hip_pair_eam(cac_hip_ctx* h, int nrho, double drho, int nr, double dr, d
     upload(c.frho_spl, fs, c.stream);
     upload(c.rhor_spl, rs, c.stream);
     upload(c.z2r_spl, zs, c.stream);
-    // FP32 table of the mixed path, staged in LDS by the pair kernels: per knot
-    // the value coefficients c3..c6 of rhor, then of z2r (derivatives are
-    // formed from them)
-    std::vector<float> t32((size_t)(nr + 1) * 8);
-    for (int m = 0; m <= nr; m++)
+    // FP32 tables of the mixed path, staged in LDS by the pair kernels: the
+    // density pass reads rhor's value coefficients c3..c6; the force pass
+    // z2r's value coefficients and the derivative coefficients c0..c2 of rhor
+    // and c2 of z2r (z2r's c0, c1 are 3 c3 / dr and 2 c4 / dr, formed in the
+    // kernel), two float4 per knot
+    std::vector<float> t32r((size_t)(nr + 1) * 4), t32f((size_t)(nr + 1) * 8);
+    for (int m = 0; m <= nr; m++) {
       for (int q = 0; q < 4; q++) {
-        t32[(size_t)m * 8 + q] = (float)rs[(size_t)m * 7 + 3 + q];
-        t32[(size_t)m * 8 + 4 + q] = (float)zs[(size_t)m * 7 + 3 + q];
+        t32r[(size_t)m * 4 + q] = (float)rs[(size_t)m * 7 + 3 + q];
+        t32f[(size_t)m * 8 + q] = (float)zs[(size_t)m * 7 + 3 + q];
       }
+      for (int q = 0; q < 3; q++) t32f[(size_t)m * 8 + 4 + q] = (float)rs[(size_t)m * 7 + q];
+      t32f[(size_t)m * 8 + 7] = (float)zs[(size_t)m * 7 + 2];
+    }
     std::vector<double> t64((size_t)(nr + 1) * 8);
     for (int m = 0; m <= nr; m++)
       for (int q = 0; q < 4; q++) {
@@ -473,7 +478,8 @@ int cac_hip_pair_eam(cac_hip_ctx* h, int nrho, double drho, int nr, double dr, d
         t64[(size_t)m * 8 + 4 + q] = zs[(size_t)m * 7 + 3 + q];
       }
     if ((size_t)(nr + 1) * 64 > 160 * 1024) return fail(h, CAC_HIP_EARG, "nr too large for the LDS spline table");
-    upload(c.tab32, t32, c.stream);
+    upload(c.tab32r, t32r, c.stream);
+    upload(c.tab32f, t32f, c.stream);
     upload(c.tab64, t64, c.stream);
     c.built = false;
     return CAC_HIP_OK;

@@ -95,11 +95,13 @@ struct DBuf {
   ~DBuf() {
     if (p) (void)hipFree(p);
   }
+  // grows only, with a quarter of headroom, so that sizes that drift from
+  // one neighbor build to the next do not reallocate every time
   void reserve(size_t count) {
     if (count <= n && p) return;
     if (p) CAC_HIP(hipFree(p));
     p = nullptr;
-    size_t c = count ? count : 1;
+    size_t c = count ? count + count / 4 : 1;
     CAC_HIP(hipMalloc(&p, c * sizeof(T)));
     n = c;
   }
@@ -181,7 +183,8 @@ struct Ctx {
   DBuf<double> tsx;
   DBuf<int> sdesc, simg, sdesc2, simg2, used, newidx;
   DBuf<int> tdesc, timg, tcell, perm, row_point, row_site, ecount, eoffset, scan_tmp;
-  DBuf<int> cell_count, cell_start, cell_cursor, occ, nearA, nearB;
+  DBuf<int> cell_start, occ, nearA, nearB, sort_keys, sort_vals;
+  DBuf<unsigned char> sort_tmp;
   DBuf<int> nnum, nlist;
   int cap = 0;
   DBuf<int> innum, ilist;
@@ -210,7 +213,8 @@ struct Ctx {
   int nrho = 0, nr = 0;
   double drho = 0, dr = 0, rdr = 0, rdrho = 0, rhomax = 0, cutmax = 0, cutforcesq = 0;
   DBuf<double> frho_spl, rhor_spl, z2r_spl;  // [(n+1)*7]
-  DBuf<float> tab32;   // per knot: rhor c3..c6, z2r c3..c6 (FP32, mixed path)
+  DBuf<float> tab32r;  // mixed density pass, per knot: rhor c3..c6 (FP32)
+  DBuf<float> tab32f;  // mixed force pass, per knot: z2r c3..c6, then rhor c0..c2 and z2r c2 (FP32)
   DBuf<double> tab64;  // the same in FP64
   int precision = 0;     // 0 FP64, 1 mixed
   int tpr = 8;           // lanes per neighbor row in the pair kernels
@@ -246,6 +250,7 @@ void build_inner(Ctx& c);     // prune the outer list to rc + delta for the forc
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
                   std::vector<int>& ucell, std::vector<int>& img);
 void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total);
+void sort_sites_by_cell(Ctx& c, const int* tcell, int S, int* perm);  // stable, by cell; fills cell_start
 
 // pair.hip
 // positions of every live site; full = also the FP64 records (always kept in

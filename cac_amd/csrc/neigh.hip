@@ -341,38 +341,6 @@ __global__ __launch_bounds__(VB) void k_virt(const double* __restrict__ x, const
   if (!FILL && t == 0) ecount[e] = total;
 }
 
-__global__ void k_cell_count(const int* __restrict__ tcell, int S, int* __restrict__ count) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < S) atomicAdd(&count[tcell[s]], 1);
-}
-
-__global__ void k_cell_scatter(const int* __restrict__ tcell, int S, const int* __restrict__ start,
-                               int* __restrict__ cursor, int* __restrict__ perm) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  const int c = tcell[s];
-  const int slot = atomicAdd(&cursor[c], 1);
-  perm[start[c] + slot] = s;
-}
-
-// the scatter above leaves each cell's sites in arbitrary order: sort every
-// cell's segment by site id so the layout is deterministic
-__global__ void k_cell_sort(const int* __restrict__ start, const int* __restrict__ count, long ncell,
-                            int* __restrict__ perm) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int* a = perm + start[c];
-  const int n = count[c];
-  for (int i = 1; i < n; i++) {
-    int v = a[i], j = i - 1;
-    while (j >= 0 && a[j] > v) {
-      a[j + 1] = a[j];
-      j--;
-    }
-    a[j + 1] = v;
-  }
-}
-
 __global__ void k_apply_perm(const int* __restrict__ perm, int S, int S_cap, Grid g, const double* __restrict__ tsx,
                              const int* __restrict__ tdesc, const int* __restrict__ timg,
                              const int* __restrict__ tcell, double4* __restrict__ site, int* __restrict__ sdesc,
@@ -590,20 +558,9 @@ void build_neighbor(Ctx& c) {
                                           nullptr, c.eoffset.get(), c.tsx.get(), c.tdesc.get(), c.timg.get(),
                                           c.tcell.get(), S_cap);
 
-  // counting sort of sites by cell
-  c.cell_count.reserve(c.ncells + 1);
-  c.cell_start.reserve(c.ncells + 1);
-  c.cell_cursor.reserve(c.ncells + 1);
-  CAC_HIP(hipMemsetAsync(c.cell_count.get(), 0, sizeof(int) * (c.ncells + 1), c.stream));
-  CAC_HIP(hipMemsetAsync(c.cell_cursor.get(), 0, sizeof(int) * (c.ncells + 1), c.stream));
-  k_cell_count<<<grid_for(S, 256), 256, 0, c.stream>>>(c.tcell.get(), S, c.cell_count.get());
-  int tot = 0;
-  scan_exclusive(c, c.cell_count.get(), c.cell_start.get(), (int)c.ncells + 1, &tot);
+  // sites ordered by cell (stable: candidate order inside a cell)
   c.perm.reserve(S);
-  k_cell_scatter<<<grid_for(S, 256), 256, 0, c.stream>>>(c.tcell.get(), S, c.cell_start.get(), c.cell_cursor.get(),
-                                                         c.perm.get());
-  k_cell_sort<<<grid_for(c.ncells, 256), 256, 0, c.stream>>>(c.cell_start.get(), c.cell_count.get(), c.ncells,
-                                                              c.perm.get());
+  sort_sites_by_cell(c, c.tcell.get(), S, c.perm.get());
   c.site.reserve(S_cap);
   c.sdesc.reserve(S_cap);
   c.simg.reserve(S_cap);

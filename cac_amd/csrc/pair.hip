@@ -88,11 +88,36 @@ __device__ __forceinline__ Sched sched(long nchunks) {
   return s;
 }
 
-template <int TPR, typename T>
-__device__ __forceinline__ T row_sum(T v) {
-#pragma unroll
-  for (int off = TPR / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, TPR);
+// Sum over the TPR lanes of a row with DPP lane moves (quad swaps, then the
+// half-row and row mirrors): a fixed tree, no LDS traffic, and every lane of
+// the row ends with the same total.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int TPR>
+__device__ __forceinline__ double row_sum(double v) {
+  if (TPR >= 2) v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  if (TPR >= 4) v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  if (TPR >= 8) v += dpp_move<0x141>(v);  // row_half_mirror
+  if (TPR >= 16) v += dpp_move<0x140>(v); // row_mirror
+  if (TPR >= 32) v += __shfl_xor(v, 16, 32);
   return v;
+}
+
+// cell-offset table of the mixed path in LDS: code (dx+2) + 5(dy+2) + 25(dz+2)
+// -> (dx, dy, dz) * cell edge, so the pair vector is (u_i - u_j) - offset
+constexpr int NOFF = 125;
+
+__device__ __forceinline__ void stage_offsets(float4* offs, const float* cs) {
+  for (int t = threadIdx.x; t < NOFF; t += blockDim.x) {
+    const int ox = t % 5 - 2, oy = (t / 5) % 5 - 2, oz = t / 25 - 2;
+    offs[t] = make_float4((float)ox * cs[0], (float)oy * cs[1], (float)oz * cs[2], 0.0f);
+  }
 }
 
 // ------------------------------------------------------------------ site refresh
@@ -199,6 +224,17 @@ __device__ __forceinline__ int knot_r(T r, T rdr, int nr, T& p) {
   return m;
 }
 
+// FP32 knot of the mixed path; r >= 0 keeps m >= 1, r < cut keeps m <= nr - 1
+// up to rounding, which the clamp covers
+__device__ __forceinline__ int knot_r32(float r, float rdr, int nr, float& p) {
+  p = r * rdr + 1.0f;
+  int m = (int)p;
+  m = m < nr - 1 ? m : nr - 1;
+  p -= (float)m;
+  p = p < 1.0f ? p : 1.0f;
+  return m;
+}
+
 __device__ __forceinline__ int knot_rho(double rho, double rdrho, int nrho, double& p) {
   p = rho * rdrho + 1.0;
   int m = (int)p;
@@ -260,10 +296,13 @@ __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __re
   extern __shared__ float4 lds_raw[];
   float4* lds32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
-  if (MIXED)
-    stage_table_strided(lds32, static_cast<const float4*>(tab), nr + 1);
-  else
+  const float4* offs = lds32 + (nr + 1);
+  if (MIXED) {
+    stage_offsets(lds32 + (nr + 1), a.cs);
+    stage_table(lds32, static_cast<const float4*>(tab), nr + 1);
+  } else {
     stage_table_strided(lds64, static_cast<const double4*>(tab), nr + 1);
+  }
   const int lane = threadIdx.x % TPR;
   const int rsub = threadIdx.x / TPR;
   const Sched s = sched(a.nchunks);
@@ -281,12 +320,14 @@ __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __re
         walk_row<TPR, U, float4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
             [&](unsigned ent, const float4& sj) {
-              float dx, dy, dz;
-              mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
+              const float4 o = offs[ent >> MIX_SHIFT];
+              const float dx = (pi.x - sj.x) - o.x;
+              const float dy = (pi.y - sj.y) - o.y;
+              const float dz = (pi.z - sj.z) - o.z;
               const float rsq = dx * dx + dy * dy + dz * dz;
               if (rsq < cut32) {
                 float p;
-                const int m = knot_r<float>(__fsqrt_rn(rsq), rdr32, nr, p);
+                const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, nr, p);
                 const float4 c = lds32[m];
                 acc += (double)(((c.x * p + c.y) * p + c.z) * p + c.w);
               }
@@ -363,7 +404,7 @@ __global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, U
 // ------------------------------------------------------------------ EAM force (pair_eam.cpp:853-1027)
 
 struct EamForceArgs {
-  const void* tab;     // LDS-staged value coefficients (float4 or double4 pairs per knot)
+  const void* tab;     // LDS-staged spline coefficients, two float4 or double4 per knot
   const double* frho;  // [(nrho+1)*7]
   const double* rho;   // per point
   double rdr, rdrho, rhomax;
@@ -378,14 +419,18 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
   extern __shared__ float4 lds_raw[];
   float4* lds32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
-  if (MIXED)
+  const float4* offs = lds32 + 2 * (q.nr + 1);
+  if (MIXED) {
+    stage_offsets(lds32 + 2 * (q.nr + 1), a.cs);
     stage_table(lds32, static_cast<const float4*>(q.tab), 2 * (q.nr + 1));
-  else
+  } else {
     stage_table(lds64, static_cast<const double4*>(q.tab), 2 * (q.nr + 1));
+  }
   const int lane = threadIdx.x % TPR;
   const int rsub = threadIdx.x / TPR;
   const Sched s = sched(a.nchunks);
   const float rdr32 = (float)q.rdr;
+  const float k3 = 3.0f * rdr32, k2 = 2.0f * rdr32;
   for (long ch = s.first; ch < s.end; ch += s.stride) {
     const int r = (int)(ch * (NT / TPR)) + rsub;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -401,23 +446,24 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
         walk_row<TPR, U, float4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
             [&](unsigned ent, const float4& sj) {
-              float dx, dy, dz;
-              mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
+              const float4 o = offs[ent >> MIX_SHIFT];
+              const float dx = (pi.x - sj.x) - o.x;
+              const float dy = (pi.y - sj.y) - o.y;
+              const float dz = (pi.z - sj.z) - o.z;
               const float rsq = dx * dx + dy * dy + dz * dz;
               if (rsq < cut32) {
-                const float rr = __fsqrt_rn(rsq);
+                const float recip = __builtin_amdgcn_rsqf(rsq);
+                const float rr = rsq * recip;
                 float p;
-                const int m = knot_r<float>(rr, rdr32, q.nr, p);
-                const float4 cr = lds32[2 * m];
-                const float4 cz = lds32[2 * m + 1];
-                const float rhop = ((3.0f * cr.x * p + 2.0f * cr.y) * p + cr.z) * rdr32;
-                const float z2p = ((3.0f * cz.x * p + 2.0f * cz.y) * p + cz.z) * rdr32;
+                const int m = knot_r32(rr, rdr32, q.nr, p);
+                const float4 cz = lds32[2 * m];      // z2r value c3..c6
+                const float4 cd = lds32[2 * m + 1];  // rhor derivative c0..c2, z2r c2
+                const float rhop = (cd.x * p + cd.y) * p + cd.z;
+                const float z2p = ((cz.x * k3) * p + cz.y * k2) * p + cd.w;
                 const float z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
-                const float recip = 1.0f / rr;
                 const float phi = z2 * recip;
-                const float phip = z2p * recip - phi * recip;
-                const float psip = (pi.w + sj.w) * rhop + phip;
-                const float fpair = -psip * recip;
+                const float phip = (z2p - phi) * recip;
+                const float fpair = -((pi.w + sj.w) * rhop + phip) * recip;
                 fx += (double)(dx * fpair);
                 fy += (double)(dy * fpair);
                 fz += (double)(dz * fpair);
@@ -696,14 +742,16 @@ static void run_eam(Ctx& c, int eflag, int vflag) {
   double* gf = c.gf.get();
   double* er = ev ? c.erow.get() : nullptr;
   const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
-  const size_t lds_rho = knot * (c.nr + 1), lds_force = 2 * knot * (c.nr + 1);
+  const size_t offs = MIXED ? sizeof(float4) * NOFF : 0;
+  const size_t lds_rho = knot * (c.nr + 1) + offs, lds_force = 2 * knot * (c.nr + 1) + offs;
   PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
-  const void* tab = MIXED ? (const void*)c.tab32.get() : (const void*)c.tab64.get();
+  const void* tab_rho = MIXED ? (const void*)c.tab32r.get() : (const void*)c.tab64.get();
+  const void* tab = MIXED ? (const void*)c.tab32f.get() : (const void*)c.tab64.get();
   {
     ScopedTimer tm(c, "eam_density");
     auto kd = k_eam_density<TPR, MIXED, NT, U>;
     const int grid = resident_grid(c, kd, NT, lds_rho, a.nchunks);
-    kd<<<grid, NT, lds_rho, c.stream>>>(a, tab, c.rdr, c.nr, c.rho.get());
+    kd<<<grid, NT, lds_rho, c.stream>>>(a, tab_rho, c.rdr, c.nr, c.rho.get());
   }
   {
     ScopedTimer tm(c, "eam_embed");
