@@ -25,6 +25,8 @@ namespace cac {
 Ctx::~Ctx() {
   if (h_red) (void)hipHostFree(h_red);
   if (h_flags) (void)hipHostFree(h_flags);
+  if (h_steps) (void)hipHostFree(const_cast<int*>(h_steps));
+  if (ev_check) (void)hipEventDestroy(ev_check);
   for (auto& row : ev)
     for (auto& e : row)
       if (e) (void)hipEventDestroy(e);
@@ -192,6 +194,18 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     CAC_HIP(hipHostMalloc((void**)&c.h_flags, sizeof(ErrorFlags), hipHostMallocDefault));
     c.d_flags.reserve(1);
     CAC_HIP(hipMemsetAsync(c.d_flags.get(), 0, sizeof(ErrorFlags), c.stream));
+    // step words of the speculative step: device copy for the gated kernels,
+    // host-mapped copy for the host
+    c.d_steps.reserve(2);
+    c.live_gen.reserve(1);
+    CAC_HIP(hipMemsetAsync(c.live_gen.get(), 0, sizeof(int), c.stream));
+    CAC_HIP(hipMemsetAsync(c.d_steps.get(), 0xFF, sizeof(int) * 2, c.stream));
+    int* hs = nullptr;
+    CAC_HIP(hipHostMalloc((void**)&hs, sizeof(int) * 4, hipHostMallocMapped | hipHostMallocCoherent));
+    hs[0] = hs[1] = -1;
+    c.h_steps = hs;
+    CAC_HIP(hipHostGetDevicePointer((void**)&c.h_steps_dev, hs, 0));
+    CAC_HIP(hipEventCreateWithFlags(&c.ev_check, hipEventDisableTiming));
     c.units = units;
     if (const char* e = std::getenv("CAC_HIP_TPR")) c.tpr = std::atoi(e);
     if (const char* e = std::getenv("CAC_HIP_GRID")) c.grid_cap = std::max(8, std::atoi(e));
@@ -699,29 +713,41 @@ int cac_hip_run(cac_hip_ctx* h, int nsteps, double dt, int thermo_every, double*
     if (th) record();
     for (int s = 0; s < nsteps; s++) {
       c.ntimestep++;
+      const int N = (int)c.ntimestep;
       const bool last = s == nsteps - 1;
-      const bool evstep = th && ((thermo_every > 0 && c.ntimestep % thermo_every == 0) || last);
-      nve_initial(c, dt);
+      const int ev = th && ((thermo_every > 0 && c.ntimestep % thermo_every == 0) || last) ? 1 : 0;
       // Neighbor::decide: the reference's trigger when due, and every step the
       // inner list's own displacement test
-      int flag = 0, inner = 0;
       c.ago++;
       const bool due = c.ago >= c.delay && c.ago % c.every == 0;
-      if (due && !c.check)
-        flag = 1;
-      else {
-        flag = check_distance(c, due && c.check, &inner);
-        if (flag && c.ago == std::max(c.every, c.delay)) c.ndanger++;
-      }
-      if (flag) {
+      if (due && !c.check) {  // rebuild without a check
+        nve_initial(c, dt);
         build_neighbor(c);
+        pair_passes(c, ev, ev, Gate());
+        step_end(c, dt, Gate());
       } else {
-        site_refresh(c, inner != 0);
-        if (inner) build_inner(c);
+        // queue the step as if no rebuild were due; its kernels return at once
+        // if the check in step_begin asks for one, and the host, which waits
+        // only for that check, then rebuilds and runs the step again
+        const Gate g{c.d_steps.get(), N};
+        step_begin(c, dt, N, due);
+        CAC_HIP(hipEventRecord(c.ev_check, c.stream));
+        site_refresh(c, false, g);
+        build_inner(c, g);
+        pair_passes(c, ev, ev, g);
+        step_end(c, dt, g);
+        CAC_HIP(hipEventSynchronize(c.ev_check));
+        if (c.h_steps[0] == N) {
+          if (c.ago == std::max(c.every, c.delay)) c.ndanger++;
+          build_neighbor(c);
+          pair_passes(c, ev, ev, Gate());
+          step_end(c, dt, Gate());
+        } else if (c.h_steps[1] == N) {
+          c.inner_builds++;
+        }
       }
-      compute_forces(c, evstep ? 1 : 0, evstep ? 1 : 0);
-      nve_final(c, dt);
-      if (evstep) record();
+      if (ev) pair_sums(c, ev, ev);
+      if (ev) record();
     }
     read_error_flags(c);
     if (c.h_flags->nan) return fail(h, CAC_HIP_ENAN, "Non-numeric positions - simulation unstable");

@@ -26,6 +26,7 @@
 
 #include <cstdint>
 #include <map>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -131,6 +132,18 @@ struct ErrorFlags {  // device-written status words
   int pad[2];
 };
 
+// Gate of the speculative no-rebuild step: the host queues a whole step
+// before it knows whether the displacement check asked for a rebuild; the
+// check writes the step number into steps[0] (rebuild) / steps[1] (re-prune
+// the inner list), and gated kernels of that step return at once on a
+// rebuild.  step < 0: not gated.
+struct Gate {
+  const int* steps = nullptr;
+  int step = -1;
+  __device__ bool skip() const { return step >= 0 && steps[0] == step; }
+  __device__ bool prune() const { return step < 0 || (steps[1] == step && steps[0] != step); }
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -221,6 +234,7 @@ struct Ctx {
   int grid_cap = 4096;   // max workgroups of the persistent pair kernels
   int pair_shape = 0;    // EAM pass shape: 0/1/2/3 = (512|1024 threads) x (4|8 entries in flight)
   int ncu = 256;         // compute units of the device
+  std::map<std::pair<const void*, size_t>, int> resident;  // workgroups per CU of a kernel at an LDS size
   int nodef_style = 0;   // 0 consistent, 1 lumped, 2 node/only
 
   // energy / virial
@@ -229,6 +243,12 @@ struct Ctx {
   DBuf<double> partials;  // block partial sums
   double* h_red = nullptr;  // pinned host scratch for reductions
   ErrorFlags* h_flags = nullptr;  // pinned mirror of the device flags
+  DBuf<int> d_steps;              // [rebuild step, prune step] written by the displacement check
+  DBuf<int> live, live_gen;       // per-site stamp of the last prune that kept it, and that prune's stamp
+  int live_serial = 0;            // stamps of ungated prunes: -1, -2, ...
+  volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
+  int* h_steps_dev = nullptr;       // device address of h_steps
+  hipEvent_t ev_check = nullptr;    // recorded after the check of a speculative step
   DBuf<ErrorFlags> d_flags;
 
   // timing
@@ -246,7 +266,7 @@ void launch_pbc(Ctx& c);
 void check_deformation(Ctx& c, bool first);
 void build_neighbor(Ctx& c);  // pbc + sites + cells + full lists
 int check_distance(Ctx& c, bool outer, int* inner);  // 1 when any point moved more than skin/2
-void build_inner(Ctx& c);     // prune the outer list to rc + delta for the force passes
+void build_inner(Ctx& c, Gate g = Gate());  // prune the outer list to rc + delta for the force passes
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
                   std::vector<int>& ucell, std::vector<int>& img);
 void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total);
@@ -255,12 +275,18 @@ void sort_sites_by_cell(Ctx& c, const int* tcell, int S, int* perm);  // stable,
 // pair.hip
 // positions of every live site; full = also the FP64 records (always kept in
 // FP64 mode; the mixed path needs them only to build or prune lists)
-void site_refresh(Ctx& c, bool full);
-void pair_compute(Ctx& c, int eflag, int vflag);
+void site_refresh(Ctx& c, bool full, Gate g = Gate());
+void pair_compute(Ctx& c, int eflag, int vflag);      // passes + nodal forces + energy/virial sums
+void pair_passes(Ctx& c, int eflag, int vflag, Gate g);  // the pair passes alone
+void pair_sums(Ctx& c, int eflag, int vflag);            // energy / virial of the last passes
 
 // integrate.hip
 void nve_initial(Ctx& c, double dt);
 void nve_final(Ctx& c, double dt);
+// speculative step: kick + drift + both displacement checks in one pass, and
+// nodal force assembly + final kick in one pass
+void step_begin(Ctx& c, double dt, int step, bool outer);
+void step_end(Ctx& c, double dt, Gate g);
 void update_centers(Ctx& c);
 void nodal_assembly(Ctx& c);
 void thermo(Ctx& c, double* ke, double* temp);

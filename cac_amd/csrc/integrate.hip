@@ -60,13 +60,14 @@ static MassTab mass_tab(const Ctx& c) {
   return t;
 }
 
+// element centres are needed only where Domain::pbc wraps elements, at a
+// neighbor build, which recomputes them from the nodes there
 void nve_initial(Ctx& c, double dt) {
   ScopedTimer tm(c, "nve");
   const double dtf = 0.5 * dt * c.ftm2v;
   k_nve_initial<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.v.get(), c.f.get(), c.ptype.get(), c.P,
                                                           mass_tab(c), dtf, dt);
   c.x_dirty = true;
-  if (c.E) update_centers(c);
 }
 
 void nve_final(Ctx& c, double dt) {
@@ -122,8 +123,7 @@ __global__ void k_nodef(const double* __restrict__ gf, double* __restrict__ f, i
   }
 }
 
-void nodal_assembly(Ctx& c) {
-  ScopedTimer tm(c, "nodef");
+static NodefTab nodef_tab(const Ctx& c) {
   NodefTab t;
   for (int g = 0; g < 8; g++)
     for (int k = 0; k < 8; k++) {
@@ -135,7 +135,123 @@ void nodal_assembly(Ctx& c) {
     t.n2g[k] = c.n2g[k];
   }
   t.style = c.nodef_style;
-  k_nodef<<<grid_for(3L * c.E, 256), 256, 0, c.stream>>>(c.gf.get(), c.f.get(), c.A, c.E, c.P, t);
+  return t;
+}
+
+void nodal_assembly(Ctx& c) {
+  ScopedTimer tm(c, "nodef");
+  k_nodef<<<grid_for(3L * c.E, 256), 256, 0, c.stream>>>(c.gf.get(), c.f.get(), c.A, c.E, c.P, nodef_tab(c));
+}
+
+// ------------------------------------------------------------------ fused step ends
+// Kick + drift of every point (fix_nve.cpp:62-131) and, on the new positions,
+// the reference's trigger (any point moved more than skin/2 since the last
+// build, neighbor.cpp:1617-1683) and the inner list's (moved more than
+// delta/2 since the last prune).  A wave with a hit writes the step number
+// into the device and host-mapped step words; every writer writes the same
+// value, so plain stores suffice.
+__global__ void k_step_begin(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ f,
+                             const int* __restrict__ ptype, const double* __restrict__ xh,
+                             const double* __restrict__ xhi, int P, MassTab mt, double dtf, double dtv, double trig,
+                             double trig_in, int outer, int step, int* __restrict__ d_steps,
+                             int* __restrict__ h_steps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int hit = 0, hit_in = 0;
+  if (p < P) {
+    const double dtfm = dtf / mt.m[ptype[p]];
+    double xn[3];
+    for (int d = 0; d < 3; d++) {
+      const size_t i = (size_t)d * P + p;
+      const double vv = v[i] + dtfm * f[i];
+      v[i] = vv;
+      xn[d] = x[i] + dtv * vv;
+      x[i] = xn[d];
+    }
+    if (outer) {
+      const double dx = xn[0] - xh[p], dy = xn[1] - xh[(size_t)P + p], dz = xn[2] - xh[2 * (size_t)P + p];
+      hit = (dx * dx + dy * dy + dz * dz > trig);
+    }
+    const double ex = xn[0] - xhi[p], ey = xn[1] - xhi[(size_t)P + p], ez = xn[2] - xhi[2 * (size_t)P + p];
+    hit_in = (ex * ex + ey * ey + ez * ez > trig_in);
+  }
+  const bool lead = (threadIdx.x & 63) == 0;
+  if (__ballot(hit) && lead) {
+    d_steps[0] = step;
+    h_steps[0] = step;
+  }
+  if (__ballot(hit_in) && lead) {
+    d_steps[1] = step;
+    h_steps[1] = step;
+  }
+}
+
+void step_begin(Ctx& c, double dt, int step, bool outer) {
+  ScopedTimer tm(c, "nve");
+  const double dtf = 0.5 * dt * c.ftm2v;
+  const double hd = 0.5 * c.inner_skin;
+  k_step_begin<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.v.get(), c.f.get(), c.ptype.get(),
+                                                         c.xhold.get(), c.xhold_in.get(), c.P, mass_tab(c), dtf, dt,
+                                                         c.triggersq, hd * hd, outer ? 1 : 0, step, c.d_steps.get(),
+                                                         c.h_steps_dev);
+  c.x_dirty = true;
+}
+
+// Nodal forces of every element (Element::compute_nodef) and the final kick of
+// every point (fix_nve.cpp:139-217): threads [0, A) kick atoms; thread A + d E + e
+// assembles dimension d of element e's nodal forces and kicks its nodes.
+__global__ void k_step_end(const double* __restrict__ gf, double* __restrict__ f, double* __restrict__ v,
+                           const int* __restrict__ ptype, int A, int E, int P, NodefTab t, MassTab mt, double dtf,
+                           Gate g) {
+  if (g.skip()) return;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < A) {
+    const double dtfm = dtf / mt.m[ptype[id]];
+    for (int d = 0; d < 3; d++) {
+      const size_t i = (size_t)d * P + id;
+      v[i] = v[i] + dtfm * f[i];
+    }
+    return;
+  }
+  const int q = id - A;
+  if (q >= 3 * E) return;
+  const int d = q / E, e = q - d * E;
+  const int E8 = NPE * E;
+  double gq[8];
+  for (int k = 0; k < 8; k++) gq[k] = gf[(size_t)d * E8 + (size_t)NPE * e + t.g2n[k]];
+  double out[NPE];
+  if (t.style == 2) {
+    for (int k = 0; k < NPE; k++) out[k] = gq[t.n2g[k]];
+  } else {
+    double fc[NPE];
+    for (int k = 0; k < NPE; k++) {
+      double s = 0.0;
+      for (int j = 0; j < 8; j++) s += t.w[j][k] * gq[j];
+      fc[k] = s;
+    }
+    if (t.style == 1) {
+      for (int k = 0; k < NPE; k++) out[k] = fc[k];
+    } else {
+      for (int a = 0; a < NPE; a++) {
+        double s = 0.0;
+        for (int b = 0; b < NPE; b++) s += t.minv[a][b] * fc[b];
+        out[a] = s;
+      }
+    }
+  }
+  const size_t base = (size_t)d * P + A + (size_t)NPE * e;
+  for (int k = 0; k < NPE; k++) {
+    f[base + k] = out[k];
+    const double dtfm = dtf / mt.m[ptype[A + (size_t)NPE * e + k]];
+    v[base + k] = v[base + k] + dtfm * out[k];
+  }
+}
+
+void step_end(Ctx& c, double dt, Gate g) {
+  ScopedTimer tm(c, "nodef");
+  const double dtf = 0.5 * dt * c.ftm2v;
+  const long n = (long)c.A + 3L * c.E;
+  k_step_end<<<grid_for(n, 256), 256, 0, c.stream>>>(c.gf.get(), c.f.get(), c.v.get(), c.ptype.get(), c.A, c.E, c.P,
+                                                     nodef_tab(c), mass_tab(c), dtf, g);
 }
 
 // ------------------------------------------------------------------ deterministic reductions

@@ -505,6 +505,10 @@ void build_neighbor(Ctx& c) {
   c.ago = 0;
   c.ncalls++;
   CAC_HIP(hipMemsetAsync(c.d_flags.get(), 0, sizeof(ErrorFlags), c.stream));
+  // step words of an earlier speculative step are spent
+  CAC_HIP(hipMemsetAsync(c.d_steps.get(), 0xFF, sizeof(int) * 2, c.stream));
+  c.h_steps[0] = c.h_steps[1] = -1;
+  if (c.E) update_centers(c);
   launch_pbc(c);
   check_deformation(c, false);
   // hold positions for check_distance (neighbor.cpp:1732-1776)
@@ -632,6 +636,9 @@ void build_neighbor(Ctx& c) {
     c.S = S2;
     c.nvirt = S2 - P;
   }
+  // no site is live for the force passes until the first prune of this layout
+  c.live.reserve(c.S);
+  CAC_HIP(hipMemsetAsync(c.live.get(), 0x80, sizeof(int) * c.S, c.stream));
   c.entries = -1;  // stored entries, counted on request
   c.built = true;
   c.inner_mode = -1;
@@ -692,17 +699,27 @@ int check_distance(Ctx& c, bool outer, int* inner) {
 
 constexpr int MIX_SHIFT = 25;
 
+// Each row also holds its point's position for the next inner-list check.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site, const int* __restrict__ scell,
-                                               const int* __restrict__ row_site, const int* __restrict__ nnum,
-                                               const unsigned* __restrict__ nlist, int R, int cap, Grid g,
-                                               double cutsq_in, int* __restrict__ innum,
-                                               unsigned* __restrict__ ilist) {
+                                               const int* __restrict__ row_site, const int* __restrict__ row_point,
+                                               const int* __restrict__ nnum, const unsigned* __restrict__ nlist,
+                                               int R, int cap, Grid g, double cutsq_in, int* __restrict__ innum,
+                                               unsigned* __restrict__ ilist, const double* __restrict__ x,
+                                               double* __restrict__ xhold_in, int* __restrict__ live, int gen,
+                                               int* __restrict__ live_gen, Gate gate) {
+  if (!gate.prune()) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *live_gen = gen;
   constexpr int PU = 4;  // 16-entry groups in flight per lane group
   const int lane = threadIdx.x & 15;
   const int gbase = (threadIdx.x & 63) & ~15;
   const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
   if (r >= R) return;
+  if (lane < 3) {
+    const size_t i = (size_t)lane * R + row_point[r];
+    xhold_in[i] = x[i];
+  }
+  if (lane == 3) live[row_site[r]] = gen;
   const int n = nnum[r];
   const int k = row_site[r];
   const double4 pk = site[k];
@@ -745,14 +762,17 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
       const unsigned long long m = __ballot(keep);
       const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
       const int rank = __popc(gm & ((1u << lane) - 1u));
-      if (keep) out[count + rank] = eout;
+      if (keep) {
+        out[count + rank] = eout;
+        live[ent[u] & SITE_MASK] = gen;
+      }
       count += __popc(gm);
     }
   }
   if (lane == 0) innum[r] = count;
 }
 
-void build_inner(Ctx& c) {
+void build_inner(Ctx& c, Gate gate) {
   ScopedTimer tm(c, "inner_build");
   const bool mixed = c.precision == 1;
   if (mixed && (long)c.S >= (1L << MIX_SHIFT)) throw HipError{hipErrorInvalidValue, "more than 2^25 sites for the mixed list"};
@@ -765,15 +785,21 @@ void build_inner(Ctx& c) {
   const int blocks = grid_for(16L * c.P, 256);
   const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
   unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());
+  // stamp of the sites the pruned lists reference: the step number for a
+  // gated prune, a fresh negative serial otherwise, so stamps never repeat
+  const int gen = gate.step >= 0 ? gate.step : --c.live_serial;
   if (mixed)
-    k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.nnum.get(), nl, c.P,
-                                                 c.cap, g, c.inner_cutsq, c.innum.get(), il);
+    k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.row_point.get(),
+                                                 c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
+                                                 c.x.get(), c.xhold_in.get(), c.live.get(), gen, c.live_gen.get(),
+                                                 gate);
   else
-    k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.nnum.get(), nl, c.P,
-                                                  c.cap, g, c.inner_cutsq, c.innum.get(), il);
-  CAC_HIP(hipMemcpyAsync(c.xhold_in.get(), c.x.get(), sizeof(double) * 3 * c.P, hipMemcpyDeviceToDevice, c.stream));
+    k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.row_point.get(),
+                                                  c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
+                                                  c.x.get(), c.xhold_in.get(), c.live.get(), gen, c.live_gen.get(),
+                                                  gate);
   c.inner_mode = c.precision;
-  c.inner_builds++;
+  if (gate.step < 0) c.inner_builds++;  // a gated prune is counted once the host knows it ran
 }
 
 // ------------------------------------------------------------------ export for parity checks

@@ -133,14 +133,24 @@ struct Frame {
   double L[3], lo[3], cs[3];
 };
 
-template <bool W64, bool W32>
+// W64: 0 = FP32 copy only, 1 = FP64 records too, 2 = FP64 records too when
+// the step re-prunes the inner list
+// Only the sites the inner lists reference (stamped by the last prune) are
+// refreshed, except on a step that builds or prunes lists, which reads all.
+template <int W64, bool W32>
 __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, UcellGrid ug,
                                                  const int* __restrict__ sdesc, const int* __restrict__ simg,
                                                  const int* __restrict__ scell, int S, int A, int P, int nucell,
-                                                 Frame fr, double4* __restrict__ site, float4* __restrict__ site32) {
+                                                 Frame fr, double4* __restrict__ site, float4* __restrict__ site32,
+                                                 const int* __restrict__ live, const int* __restrict__ live_gen,
+                                                 int all, Gate g) {
 #pragma clang fp contract(off)
+  if (g.skip()) return;
+  const bool pruning = g.step >= 0 && g.prune();
+  const bool w64 = W64 == 1 || (W64 == 2 && pruning);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S) return;
+  if (!all && !pruning && live[k] != *live_gen) return;
   const int d = sdesc[k];
   double y[3];
   if (d >= 0) {
@@ -161,7 +171,7 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, U
   double sh[3];
   img_shift((unsigned)simg[k], fr.L, sh[0], sh[1], sh[2]);
   for (int c = 0; c < 3; c++) y[c] = y[c] + sh[c];
-  if (W64) site[k] = make_double4(y[0], y[1], y[2], 0.0);
+  if (w64) site[k] = make_double4(y[0], y[1], y[2], 0.0);
   if (W32) {
     const int cc = scell[k];
     site32[k] = make_float4((float)(y[0] - (fr.lo[0] + cell_ix(cc) * fr.cs[0])),
@@ -188,7 +198,7 @@ static Frame frame_of(const Ctx& c) {
   return fr;
 }
 
-void site_refresh(Ctx& c, bool full) {
+void site_refresh(Ctx& c, bool full, Gate g) {
   ScopedTimer tm(c, "site_refresh");
   c.x_dirty = false;
   const bool mixed = c.precision == 1;
@@ -197,17 +207,22 @@ void site_refresh(Ctx& c, bool full) {
   const UcellGrid ug = ucell_grid(c);
   const int nuc = c.nucell > 0 ? c.nucell : 1;
   const int grid = grid_for(c.S, 256);
+  float4* s32 = c.site32.get();
+  const int all = full ? 1 : 0;
+  const int* lv = c.live.get();
+  const int* lg = c.live_gen.get();
   if (!mixed)
-    k_refresh<true, false><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(),
-                                                       c.scell.get(), c.S, c.A, c.P, nuc, fr, c.site.get(), nullptr);
+    k_refresh<1, false><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(), c.scell.get(), c.S,
+                                                    c.A, c.P, nuc, fr, c.site.get(), nullptr, lv, lg, all, g);
   else if (full)
-    k_refresh<true, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(),
-                                                      c.scell.get(), c.S, c.A, c.P, nuc, fr, c.site.get(),
-                                                      c.site32.get());
+    k_refresh<1, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(), c.scell.get(), c.S,
+                                                   c.A, c.P, nuc, fr, c.site.get(), s32, lv, lg, all, g);
+  else if (g.step >= 0)
+    k_refresh<2, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(), c.scell.get(), c.S,
+                                                   c.A, c.P, nuc, fr, c.site.get(), s32, lv, lg, all, g);
   else
-    k_refresh<false, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(),
-                                                       c.scell.get(), c.S, c.A, c.P, nuc, fr, nullptr,
-                                                       c.site32.get());
+    k_refresh<0, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(), c.scell.get(), c.S,
+                                                   c.A, c.P, nuc, fr, nullptr, s32, lv, lg, all, g);
 }
 
 // ------------------------------------------------------------------ spline helpers
@@ -292,7 +307,8 @@ __device__ __forceinline__ void walk_row(const unsigned* __restrict__ row, int n
 // the density pass stages only the rho(r) half of the table (knot m at m)
 template <int TPR, bool MIXED, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __restrict__ tab, double rdr, int nr,
-                                                    double* __restrict__ rho_out) {
+                                                    double* __restrict__ rho_out, Gate g) {
+  if (g.skip()) return;
   extern __shared__ float4 lds_raw[];
   float4* lds32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
@@ -377,10 +393,14 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, UcellGrid ug,
                                                const int* __restrict__ sdesc, int S, int A, int nucell,
                                                const double* __restrict__ frho, double rdrho, int nrho,
-                                               double4* __restrict__ site, float4* __restrict__ site32) {
+                                               double4* __restrict__ site, float4* __restrict__ site32,
+                                               const int* __restrict__ live, const int* __restrict__ live_gen,
+                                               Gate g) {
 #pragma clang fp contract(off)
+  if (g.skip()) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S) return;
+  if (live[k] != *live_gen) return;  // no inner list reads this site
   const int d = sdesc[k];
   double rj;
   if (d >= 0) {
@@ -415,7 +435,8 @@ struct EamForceArgs {
 template <int TPR, bool MIXED, bool EV, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, double* __restrict__ f,
                                                   double* __restrict__ gf, int P, int E8,
-                                                  double* __restrict__ erow) {
+                                                  double* __restrict__ erow, Gate g) {
+  if (g.skip()) return;
   extern __shared__ float4 lds_raw[];
   float4* lds32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
@@ -575,7 +596,9 @@ struct LjArgs {
 
 template <int TPR, bool MIXED, bool EV>
 __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* __restrict__ f,
-                                                 double* __restrict__ gf, int P, int E8, double* __restrict__ erow) {
+                                                 double* __restrict__ gf, int P, int E8, double* __restrict__ erow,
+                                                 Gate g) {
+  if (g.skip()) return;
   const int lane = threadIdx.x % TPR;
   const int rsub = threadIdx.x / TPR;
   const Sched s = sched(a.nchunks);
@@ -728,15 +751,18 @@ static void allow_lds(K kernel, size_t bytes) {
 // workgroup waits for another to finish before it starts its share.
 template <typename K>
 static int resident_grid(Ctx& c, K kernel, int nt, size_t lds, long nchunks) {
-  allow_lds(kernel, lds);
-  int per_cu = 0;
-  CAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), nt, lds));
-  if (per_cu < 1) per_cu = 1;
+  const void* kp = reinterpret_cast<const void*>(kernel);
+  int& per_cu = c.resident[{kp, lds}];
+  if (per_cu == 0) {
+    allow_lds(kernel, lds);
+    CAC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, nt, lds));
+    if (per_cu < 1) per_cu = 1;
+  }
   return pair_grid(nchunks, std::min<long>((long)per_cu * c.ncu, c.grid_cap));
 }
 
 template <int TPR, bool MIXED, int NT, int U>
-static void run_eam(Ctx& c, int eflag, int vflag) {
+static void run_eam(Ctx& c, int eflag, int vflag, Gate g) {
   const bool ev = eflag || vflag;
   const int E8 = NPE * c.E;
   double* gf = c.gf.get();
@@ -751,7 +777,7 @@ static void run_eam(Ctx& c, int eflag, int vflag) {
     ScopedTimer tm(c, "eam_density");
     auto kd = k_eam_density<TPR, MIXED, NT, U>;
     const int grid = resident_grid(c, kd, NT, lds_rho, a.nchunks);
-    kd<<<grid, NT, lds_rho, c.stream>>>(a, tab_rho, c.rdr, c.nr, c.rho.get());
+    kd<<<grid, NT, lds_rho, c.stream>>>(a, tab_rho, c.rdr, c.nr, c.rho.get(), g);
   }
   {
     ScopedTimer tm(c, "eam_embed");
@@ -759,7 +785,8 @@ static void run_eam(Ctx& c, int eflag, int vflag) {
     k_embed<MIXED><<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.rho.get(), ucell_grid(c), c.sdesc.get(), c.S, c.A, nuc,
                                                              c.frho_spl.get(), c.rdrho, c.nrho,
                                                              MIXED ? nullptr : c.site.get(),
-                                                             MIXED ? c.site32.get() : nullptr);
+                                                             MIXED ? c.site32.get() : nullptr, c.live.get(),
+                                                             c.live_gen.get(), g);
   }
   EamForceArgs q;
   q.tab = tab;
@@ -776,16 +803,16 @@ static void run_eam(Ctx& c, int eflag, int vflag) {
   if (ev) {
     auto kf = k_eam_force<TPR, MIXED, true, NT, U>;
     const int grid = resident_grid(c, kf, NT, lds_force, a.nchunks);
-    kf<<<grid, NT, lds_force, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+    kf<<<grid, NT, lds_force, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er, g);
   } else {
     auto kf = k_eam_force<TPR, MIXED, false, NT, U>;
     const int grid = resident_grid(c, kf, NT, lds_force, a.nchunks);
-    kf<<<grid, NT, lds_force, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+    kf<<<grid, NT, lds_force, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er, g);
   }
 }
 
 template <int TPR>
-static void launch_pair(Ctx& c, int eflag, int vflag) {
+static void launch_pair(Ctx& c, int eflag, int vflag, Gate g) {
   const bool ev = eflag || vflag;
   const bool mixed = c.precision == 1;
   const int E8 = NPE * c.E;
@@ -795,16 +822,16 @@ static void launch_pair(Ctx& c, int eflag, int vflag) {
     // workgroup size and entries in flight per lane of the EAM passes
     if (mixed) {
       switch (c.pair_shape) {
-        case 1: run_eam<TPR, true, 1024, 4>(c, eflag, vflag); break;
-        case 2: run_eam<TPR, true, 512, 8>(c, eflag, vflag); break;
-        case 3: run_eam<TPR, true, 1024, 8>(c, eflag, vflag); break;
-        default: run_eam<TPR, true, 512, 4>(c, eflag, vflag); break;
+        case 1: run_eam<TPR, true, 1024, 4>(c, eflag, vflag, g); break;
+        case 2: run_eam<TPR, true, 512, 8>(c, eflag, vflag, g); break;
+        case 3: run_eam<TPR, true, 1024, 8>(c, eflag, vflag, g); break;
+        default: run_eam<TPR, true, 512, 4>(c, eflag, vflag, g); break;
       }
     } else {
       switch (c.pair_shape) {
         case 2:
-        case 3: run_eam<TPR, false, 1024, 8>(c, eflag, vflag); break;
-        default: run_eam<TPR, false, 1024, 4>(c, eflag, vflag); break;
+        case 3: run_eam<TPR, false, 1024, 8>(c, eflag, vflag, g); break;
+        default: run_eam<TPR, false, 1024, 4>(c, eflag, vflag, g); break;
       }
     }
   } else {
@@ -814,11 +841,11 @@ static void launch_pair(Ctx& c, int eflag, int vflag) {
     auto kl = mixed ? (ev ? k_lj_force<TPR, true, true> : k_lj_force<TPR, true, false>)
                     : (ev ? k_lj_force<TPR, false, true> : k_lj_force<TPR, false, false>);
     const int grid = resident_grid(c, kl, PB, 0, a.nchunks);
-    kl<<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er);
+    kl<<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er, g);
   }
 }
 
-void pair_compute(Ctx& c, int eflag, int vflag) {
+void pair_passes(Ctx& c, int eflag, int vflag, Gate g) {
   const bool ev = eflag || vflag;
   if (ev) c.erow.reserve((size_t)7 * c.P);
   if (c.inner_mode != c.precision) {  // precision or delta changed since the last prune
@@ -826,19 +853,26 @@ void pair_compute(Ctx& c, int eflag, int vflag) {
     build_inner(c);
   }
   switch (c.tpr) {
-    case 4: launch_pair<4>(c, eflag, vflag); break;
-    case 8: launch_pair<8>(c, eflag, vflag); break;
-    case 32: launch_pair<32>(c, eflag, vflag); break;
-    default: launch_pair<16>(c, eflag, vflag); break;
+    case 4: launch_pair<4>(c, eflag, vflag, g); break;
+    case 8: launch_pair<8>(c, eflag, vflag, g); break;
+    case 32: launch_pair<32>(c, eflag, vflag, g); break;
+    default: launch_pair<16>(c, eflag, vflag, g); break;
   }
+}
+
+void pair_sums(Ctx& c, int eflag, int vflag) {
+  if (!eflag && !vflag) return;
+  double out[7];
+  reduce_rows(c, c.erow.get(), c.P, 7, out);
+  if (eflag) c.eng_vdwl = out[0];
+  if (vflag)
+    for (int i = 0; i < 6; i++) c.virial[i] = out[1 + i];
+}
+
+void pair_compute(Ctx& c, int eflag, int vflag) {
+  pair_passes(c, eflag, vflag, Gate());
   if (c.E) nodal_assembly(c);
-  if (ev) {
-    double out[7];
-    reduce_rows(c, c.erow.get(), c.P, 7, out);
-    if (eflag) c.eng_vdwl = out[0];
-    if (vflag)
-      for (int i = 0; i < 6; i++) c.virial[i] = out[1 + i];
-  }
+  pair_sums(c, eflag, vflag);
 }
 
 }  // namespace cac
