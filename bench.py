@@ -1,24 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the CAC force + neighbor hot path on one MI355X.
 
-Workload (default): the 8x-volume mixed-precision EAM CAC configuration:
-255,879 core atoms + 13,985 Hex elements of 9^3 sites (2x2x2 Gauss points),
-367,759 atom-equivalents, 10.45 M represented sites, Cu-like analytic EAM
-tabulated as setfl (Nr = Nrho = 2000), skin 2.0 A, 300 K, dt = 1 fs.
+Workload (default, cac_eam_x8): 255,879 core atoms + 13,985 Hex elements of
+9^3 unit cells (2x2x2 Gauss points at the corners), 367,759 atom-equivalents
+(atoms + Gauss points), 10.45 M represented sites, Cu-like analytic EAM
+tabulated as setfl (Nr = Nrho = 2000, cut 4.95 A), skin 2.0 A, 300 K, dt 1 fs,
+neighbor every 1 delay 10 check yes (the reference's defaults).
 
-A step is one velocity-Verlet step of the whole model: NVE half kick + drift,
+A step is one velocity-Verlet step of the whole model: half kick + drift,
 neighbor decision (and rebuild when due), site refresh, EAM density /
-embedding / force passes, nodal force assembly, NVE half kick.  The metric is
-atom-equivalent steps per second (real atoms + Gauss points) in mixed
-precision; the FP64 path is run on the same deck for the force error and the
-speedup.  Inputs are resident in HBM before the timed region.
+embedding / force passes, nodal force assembly, half kick.  The metric is
+atom-equivalent steps per second in mixed precision (FP32 pair terms, FP64
+positions and sums); the FP64 path runs on the same deck for the force error
+and the speedup.  Inputs are resident in HBM before the timed region.
 
-Prints one JSON line.
+Prints one JSON line.  `--profile-run` instead runs only the mixed model for
+--steps steps and then a 1 GiB device-to-device copy, the calibration kernel
+of the HBM traffic counters; it is the program the committed rocprofv3
+summaries were taken on.
 """
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -28,8 +33,11 @@ sys.path.insert(0, ROOT)
 WORKLOADS = {
     # name: (box cells, core cells, ncell, description)
     "cac_eam_x8": ((14, 16, 16), 351, 9, "255,879 atoms + 13,985 Hex(9^3) elements, EAM, mixed vs FP64"),
-    "cac_eam": ((7, 8, 8), 44, 9, "32,076 atoms + 1,748 Hex(9^3) elements, EAM"),
+    "cac_eam": ((7, 8, 8), 44, 9, "32,076 atoms + 1,748 Hex(9^3) elements, EAM, mixed vs FP64"),
 }
+SKIN, DT = 2.0, 0.001
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "eam_force_mixed_cac_eam_x8_pmc.md")
 
 
 def force_vector(m):
@@ -48,17 +56,56 @@ def timed_run(m, steps, warmup, dt):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(deck, tab, skin, dt, nsteps):
+def cpu_baseline(deck, tab, nsteps):
+    """The oracle (one-thread C restatement of the reference's algorithm) on the same deck."""
     from oracle import oracle as O
-    o = O.Oracle(deck, "eam", skin=skin, eam_tables=tab)
+    o = O.Oracle(deck, "eam", skin=SKIN, eam_tables=tab)
     t0 = time.perf_counter()
     o.setup(eflag=0, vflag=0)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    o.run(nsteps, dt)
+    o.run(nsteps, DT)
     t_run = time.perf_counter() - t0
     o.close()
     return t_setup, t_run
+
+
+def force_bytes(P, inner_mean, live_sites):
+    """Algorithmic HBM bytes of one mixed EAM force launch.
+
+    Per row: its site index, point index and inner-list length (12 B) and the
+    FP64 force it writes (24 B); per inner-list entry its 4-byte word; per
+    site the inner lists reference, its FP32 record (x, y, z, F') read once.
+    """
+    return P * (36.0 + 4.0 * inner_mean) + 16.0 * live_sites
+
+
+def traffic_from_summary():
+    """HBM bytes per force launch measured with FETCH_SIZE / WRITE_SIZE (committed summary), or None."""
+    try:
+        text = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
+    if not m:
+        return None
+    return json.loads(m.group(1))
+
+
+def hip_copy_calibration(nbytes=1 << 30):
+    """One device-to-device copy of nbytes through the HIP runtime (a plain copy kernel)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemset failed")
+    if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def main():
@@ -72,9 +119,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--inner-skin", type=float, default=-1.0,
                     help="prune margin delta of the inner list (A); < 0 takes 0.25 * skin")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="mixed model only, --steps steps, then a 1 GiB copy (program of the rocprofv3 summaries)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this build runs on one GPU; runs on several GPUs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
@@ -84,20 +133,34 @@ def main():
     from cac_amd import decks
 
     box, core, nc, desc = WORKLOADS[args.workload]
-    skin, dt = 2.0, 0.001
     tab = decks.cu_eam_tables()
     deck = decks.cac_eam(box_cells=box, core_cells=core, ncell=nc, jitter=0.02, T=300.0, seed=2024)
     P = deck.atom_equivalents
 
+    if args.profile_run:
+        m = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.MIXED)
+        if args.inner_skin >= 0:
+            m.set_inner_skin(args.inner_skin)
+        m.setup()
+        print(f"profile-run: {args.workload} set up", flush=True)
+        m.run(args.steps, DT, thermo=False)
+        m.sync()
+        _, nbar_in, _, live = m.inner_stats()
+        print(f"profile-run: {args.steps} mixed steps done; inner mean {nbar_in:.2f}, live sites {live}, "
+              f"force bytes/launch {force_bytes(P, nbar_in, live):.0f}", flush=True)
+        hip_copy_calibration()
+        print("profile-run: calibration copy of 1 GiB done", flush=True)
+        return
+
     # FP64 and mixed models on the same deck
-    m64 = cac_amd.Model(deck, "eam", skin=skin, eam_tables=tab, precision=cac_amd.FP64)
-    m64.setup()
-    f64 = force_vector(m64)
-    e64, _ = m64.energy()
-    mmx = cac_amd.Model(deck, "eam", skin=skin, eam_tables=tab, precision=cac_amd.MIXED)
+    m64 = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.FP64)
+    mmx = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.MIXED)
     if args.inner_skin >= 0:
         m64.set_inner_skin(args.inner_skin)
         mmx.set_inner_skin(args.inner_skin)
+    m64.setup()
+    f64 = force_vector(m64)
+    e64, _ = m64.energy()
     mmx.setup()
     fmx = force_vector(mmx)
     emx, _ = mmx.energy()
@@ -107,43 +170,41 @@ def main():
     rms_rel_err = float(np.sqrt(np.mean(df ** 2)) / np.sqrt(np.mean(fmag ** 2)))
     info = mmx.neigh_info()
     nbar = info["entries"] / info["rows"]
-    _, nbar_in, delta = mmx.inner_stats()
 
     b0, ib0 = mmx.neigh_stats()[0], mmx.inner_stats()[0]
-    t_mx = timed_run(mmx, args.steps, args.warmup, dt)
-    t_64 = timed_run(m64, args.steps, args.warmup, dt)
+    t_mx = timed_run(mmx, args.steps, args.warmup, DT)
     nsteps_mx = args.steps + args.warmup
     rebuilds = (mmx.neigh_stats()[0] - b0) / nsteps_mx
     prunes = (mmx.inner_stats()[0] - ib0) / nsteps_mx
+    t_64 = timed_run(m64, args.steps, args.warmup, DT)
     value = P * args.steps / t_mx
     value64 = P * args.steps / t_64
 
-    # per-kernel durations from hipEvents on the context stream (separate short run)
-    mmx.enable_timing(True)
-    mmx.run(args.profile_steps, dt, thermo=False)
+    # per-kernel durations from hipEvents on the context's stream (a separate
+    # short run, so the events do not perturb the timed one)
     names = ("eam_density", "eam_embed", "eam_force", "site_refresh", "nodef", "nve", "neigh_build", "inner_build")
+    mmx.enable_timing(True)
+    mmx.run(args.profile_steps, DT, thermo=False)
     kt = {k: mmx.kernel_time(k) for k in names}
     mmx.enable_timing(False)
     m64.enable_timing(True)
-    m64.run(args.profile_steps, dt, thermo=False)
+    m64.run(args.profile_steps, DT, thermo=False)
     kt64 = {k: m64.kernel_time(k) for k in ("eam_density", "eam_embed", "eam_force")}
     m64.enable_timing(False)
+    _, nbar_in, delta, live = mmx.inner_stats()
 
-    # dominant kernel: EAM force pass (mixed).  Algorithmic bytes per launch:
-    # per row its site index, point index and inner-list length (12) and the
-    # FP64 force write (24); 4 bytes per inner-list entry; each site record
-    # (x, y, z, F' as FP32, 16 bytes) read once.
+    # dominant kernel: the mixed EAM force pass
     force_us = kt["eam_force"][0]
-    bytes_force = P * (36.0 + 4.0 * nbar_in) + 16.0 * info["sites"]
+    bytes_force = force_bytes(P, nbar_in, live)
     achieved = bytes_force / (force_us * 1e-6) / 1e9
-    peak = 8000.0
+    tr = traffic_from_summary() if args.workload == "cac_eam_x8" else None
 
     cpu = None
     if not args.no_cpu_baseline:
-        t_setup, t_run = cpu_baseline(deck, tab, skin, dt, args.cpu_steps)
+        t_setup, t_run = cpu_baseline(deck, tab, args.cpu_steps)
         cpu = {"value": P * args.cpu_steps / t_run, "unit": "atom-eq steps/s", "cores": 1, "kind": "port",
                "sample": f"oracle (CPU restatement, one thread) running {args.cpu_steps} NVE steps of this deck "
-                         f"after setup; setup (neighbor build + first force) {t_setup:.1f} s excluded"}
+                         f"after its setup; setup (neighbor build + first force, {t_setup:.1f} s) excluded"}
 
     line = {
         "metric": "atom-steps/s (mixed vs FP64 @ <1e-5 rel force err)",
@@ -156,22 +217,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair terms / fp64 accumulation",
+        "dtype": "fp32 pair terms / fp64 positions and sums",
         "data": "synthetic (seeded FCC Cu CAC deck, analytic EAM tabulated as setfl)",
         "config": {"workload": args.workload, "description": desc, "atoms": deck.natoms,
                    "elements": deck.nelements, "atom_equivalents": P, "represented_sites": deck.represented_sites,
-                   "skin": skin, "dt_ps": dt, "neighbor_mean_full": round(nbar, 2), "sites": info["sites"],
-                   "inner_delta": delta, "inner_mean": round(nbar_in, 2),
+                   "skin": SKIN, "dt_ps": DT, "neighbor_mean_full": round(nbar, 2), "sites": info["sites"],
+                   "inner_delta": delta, "inner_mean": round(nbar_in, 2), "inner_live_sites": live,
                    "outer_builds_per_step": round(rebuilds, 4), "inner_prunes_per_step": round(prunes, 4)},
         "fp64": {"value": value64, "ms_per_step": 1000.0 * t_64 / args.steps,
                  "speedup_mixed_over_fp64": value / value64},
         "force_error_vs_fp64": {"max_rel": max_rel_err, "rms_rel": rms_rel_err,
                                 "energy_rel": abs(emx - e64) / abs(e64)},
         "kernels_us": {k: round(v[0], 3) for k, v in kt.items() if v[1]},
-        "kernels_us_fp64": {k: round(v[0], 3) for k, v in kt64.items()},
-        "roofline": {"bound": "hbm", "kernel": "eam_force (mixed)", "achieved": achieved, "peak": peak,
-                     "unit": "GB/s", "frac": achieved / peak, "traffic": None,
-                     "bytes_per_launch": bytes_force},
+        "kernels_us_fp64": {k: round(v[0], 3) for k, v in kt64.items() if v[1]},
+        "roofline": {"bound": "hbm", "kernel": "eam_force (mixed)", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": tr["bytes_per_launch"] if tr else None,
+                     "algorithmic_bytes": bytes_force,
+                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if tr else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

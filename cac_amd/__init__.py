@@ -84,7 +84,7 @@ def lib():
             "cac_hip_enable_timing": [vp, i],
             "cac_hip_kernel_time": [vp, C.c_char_p, C.POINTER(d), C.POINTER(C.c_long)],
             "cac_hip_neigh_stats": [vp, C.POINTER(i), C.POINTER(i)],
-            "cac_hip_inner_stats": [vp, C.POINTER(C.c_long), C.POINTER(d), C.POINTER(d)],
+            "cac_hip_inner_stats": [vp, C.POINTER(C.c_long), C.POINTER(d), C.POINTER(d), C.POINTER(i)],
             "cac_hip_set_inner_skin": [vp, d],
         }
         for name, args in sig.items():
@@ -222,10 +222,10 @@ class Model:
         return a.value, b.value
 
     def inner_stats(self):
-        """(prunes of the inner list, mean inner entries per point, delta)"""
-        n, m, dl = C.c_long(), C.c_double(), C.c_double()
-        self._chk(self.L.cac_hip_inner_stats(self.h, C.byref(n), C.byref(m), C.byref(dl)))
-        return n.value, m.value, dl.value
+        """(prunes of the inner list, mean inner entries per point, delta, sites the inner lists reference)"""
+        n, m, dl, lv = C.c_long(), C.c_double(), C.c_double(), C.c_int()
+        self._chk(self.L.cac_hip_inner_stats(self.h, C.byref(n), C.byref(m), C.byref(dl), C.byref(lv)))
+        return n.value, m.value, dl.value, lv.value
 
     def set_inner_skin(self, delta):
         self._chk(self.L.cac_hip_set_inner_skin(self.h, float(delta)))

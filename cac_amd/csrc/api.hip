@@ -858,8 +858,20 @@ int cac_hip_neigh_stats(cac_hip_ctx* h, int* ncalls, int* ndanger) {
   });
 }
 
-int cac_hip_inner_stats(cac_hip_ctx* h, long* builds, double* mean_entries, double* delta) {
+int cac_hip_inner_stats(cac_hip_ctx* h, long* builds, double* mean_entries, double* delta, int* live_sites) {
   return guarded(h, [&](Ctx& c) -> int {
+    if (live_sites) {
+      int n = 0;
+      if (c.built && c.S && c.inner_mode >= 0) {
+        std::vector<int> st(c.S);
+        int gen = 0;
+        CAC_HIP(hipStreamSynchronize(c.stream));
+        CAC_HIP(hipMemcpy(st.data(), c.live.get(), sizeof(int) * c.S, hipMemcpyDeviceToHost));
+        CAC_HIP(hipMemcpy(&gen, c.live_gen.get(), sizeof(int), hipMemcpyDeviceToHost));
+        for (int v : st) n += v == gen;
+      }
+      *live_sites = n;
+    }
     if (builds) *builds = c.inner_builds;
     if (delta) *delta = c.inner_skin;
     if (mean_entries) *mean_entries = c.P && c.inner_mode >= 0 ? (double)count_entries(c, c.innum) / c.P : 0.0;
