@@ -310,3 +310,119 @@ def test_cac_force_is_deterministic():
         o.run(20, 0.001)
         res.append(o.elements()["nodex"])
     assert np.array_equal(res[0], res[1])
+
+
+# ---------------------------------------------------------------- composite CAC force
+
+@pytest.mark.parametrize("nc,box,core", [(3, (3, 3, 3), 8), (5, (2, 2, 2), 4)])
+def test_cac_eam_matches_all_pairs_definition(nc, box, core):
+    """Forces, densities and energy of the half/bin/newton machinery (asymmetric
+    pair storage, reactions only on atoms and Gauss points, per-pair F' of
+    interpolated atoms from interpolated nodal densities, nodal energy weights)
+    equal an all-pairs evaluation from the definitions."""
+    d = decks.cac_eam(box_cells=box, core_cells=core, ncell=nc, jitter=0.06, T=0.0, seed=17 + nc)
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB)
+    o.setup()
+    fa, fg, ra, rn, e = brute.cac_eam(d, TAB)
+    _, _, f = o.atoms()
+    el = o.elements()
+    ar, nr = o.rho()
+    eo, _ = o.energy()
+    scale = np.abs(fg).max()
+    assert np.abs(f - fa).max() < 1e-10 * scale
+    assert np.abs(el["gaussf"] - fg).max() < 1e-10 * scale
+    assert np.allclose(ar, ra, rtol=1e-12) and np.allclose(nr, rn, rtol=1e-12)
+    assert eo == pytest.approx(e, rel=1e-12)
+
+
+# ---------------------------------------------------------------- thermo closed forms
+
+def test_thermo_uniform_translation():
+    """A uniform velocity V on every atom and node: the consistent mass matrix
+    sums to one per element, so KE = 1/2 m V^2 (atoms + nucell * elements) and
+    T = m V^2 / (3 k_B) whatever the geometry (compute_temp.cpp:96-150)."""
+    d = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=0.0, seed=5)
+    V = np.array([0.7, -1.1, 0.4])  # A/ps
+    d.v = np.tile(V, (d.natoms, 1))
+    d.nodev = np.tile(V, (d.nelements, 8, 1))
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB)
+    o.setup()
+    th = o.thermo()
+    mvv2e, boltz = 1.0364269e-4, 8.617343e-5
+    m = 63.546
+    n = d.natoms + 27 * d.nelements
+    ke = 0.5 * m * float(V @ V) * n * mvv2e
+    assert th["ke"] == pytest.approx(ke, rel=1e-12)
+    assert th["temp"] == pytest.approx(m * float(V @ V) * mvv2e / (3.0 * boltz), rel=1e-12)
+
+
+def test_pressure_of_a_dilute_gas():
+    """Atoms farther apart than the cutoff: no virial, so P = N k_B T / V in
+    pressure units (compute_pressure.cpp:140-180)."""
+    rng = np.random.default_rng(3)
+    g = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3) * 8.0 + 1.0
+    v = rng.normal(0, 3.0, g.shape)
+    d = decks.Deck(units="metal", boxlo=np.zeros(3), boxhi=np.full(3, 32.0), masses=np.array([63.546]), x=g, v=v,
+                   atype=np.ones(len(g), dtype=np.int32))
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB)
+    o.setup()
+    th = o.thermo()
+    boltz, nktv2p = 8.617343e-5, 1.6021765e6
+    assert th["pe"] == 0.0
+    assert th["press"] == pytest.approx(len(g) * boltz * th["temp"] / 32.0 ** 3 * nktv2p, rel=1e-12)
+
+
+# ---------------------------------------------------------------- rebuild trigger and periodic wrap
+
+def _dilute_gas(u, n=4, spacing=10.0):
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) * spacing + 1.0
+    v = np.zeros_like(g)
+    v[5] = u
+    L = n * spacing
+    return decks.Deck(units="metal", boxlo=np.zeros(3), boxhi=np.full(3, L), masses=np.array([63.546]), x=g, v=v,
+                      atype=np.ones(len(g), dtype=np.int32))
+
+
+@pytest.mark.parametrize("delay,expect", [(0, 6), (10, 5)])
+def test_rebuild_schedule_ballistic(delay, expect):
+    """One free atom moving 0.07 A per step, skin 1.0 A: the trigger (moved more
+    than skin/2 since the last build, neighbor.cpp:1617-1683) first holds 8
+    steps after a build, so with delay 0 a build lands every 8 steps (6 in 50
+    steps); with delay 10 the first check after a build is 10 steps on and
+    already trips, so every build is a dangerous one (5 in 50)."""
+    dt = 0.001
+    d = _dilute_gas(np.array([70.0, 0.0, 0.0]))
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB, delay=delay)
+    o.setup()
+    o.run(50, dt)
+    ncalls, ndanger = o.neigh_stats()
+    assert ncalls == expect
+    assert ndanger == (expect if delay else 0)
+    x, _, _ = o.atoms()
+    x0 = d.x[5] + d.v[5] * 50 * dt
+    assert np.allclose(x[5], np.mod(x0, d.boxhi), atol=1e-12)
+
+
+def test_periodic_wrap_moves_elements_rigidly():
+    """A perfect CAC lattice translating at a uniform velocity feels no force;
+    after crossing the box every atom sits at its wrapped ballistic position and
+    every element at its ballistic position shifted by whole box lengths chosen
+    by its centre (domain.cpp:671-737)."""
+    d = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.0, T=0.0, jitter_atoms=False)
+    V = np.array([900.0, -400.0, 250.0])  # A/ps: the box is crossed within the run
+    d.v = np.tile(V, (d.natoms, 1))
+    d.nodev = np.tile(V, (d.nelements, 8, 1))
+    dt, n = 0.001, 60
+    o = O.Oracle(d, "eam", skin=2.0, eam_tables=TAB, delay=0)
+    o.setup()
+    o.run(n, dt)
+    L = d.boxhi - d.boxlo
+    x, _, f = o.atoms()
+    el = o.elements()
+    assert np.abs(f).max() < 1e-9 and np.abs(el["gaussf"]).max() < 1e-9
+    xa = d.x + V * n * dt
+    assert np.allclose(x, d.boxlo + np.mod(xa - d.boxlo, L), atol=1e-9)
+    nb = d.nodex + V * n * dt
+    c = nb.mean(axis=1)
+    k = np.floor((c - d.boxlo) / L)
+    assert np.allclose(el["nodex"], nb - k[:, None, :] * L, atol=1e-9)
