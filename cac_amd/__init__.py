@@ -86,6 +86,8 @@ def lib():
             "cac_hip_neigh_stats": [vp, C.POINTER(i), C.POINTER(i)],
             "cac_hip_inner_stats": [vp, C.POINTER(C.c_long), C.POINTER(d), C.POINTER(d), C.POINTER(i)],
             "cac_hip_set_inner_skin": [vp, d],
+            "cac_hip_sync_positions": [vp, vp, vp],
+            "cac_hip_download_forces": [vp, vp, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -229,6 +231,27 @@ class Model:
 
     def set_inner_skin(self, delta):
         self._chk(self.L.cac_hip_set_inner_skin(self.h, float(delta)))
+
+    def sync_positions(self, x, nodex=None):
+        """Copy host positions in ([A,3], [E,8,3]) without touching the lists."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nx = np.ascontiguousarray(nodex, dtype=np.float64) if nodex is not None else None
+        self._chk(self.L.cac_hip_sync_positions(self.h, x.ctypes.data if x.size else None,
+                                                nx.ctypes.data if nx is not None and nx.size else None))
+
+    def decide(self):
+        """Neighbor::decide on the current positions: True when a rebuild is due."""
+        f = C.c_int()
+        self._chk(self.L.cac_hip_neigh_decide(self.h, C.byref(f)))
+        return bool(f.value)
+
+    def forces(self):
+        """(atom forces [A,3], nodal forces [E,8,3], Gauss-cell forces [E,8,3])"""
+        A, E = self.deck.natoms, self.deck.nelements
+        f, nf, gf = np.zeros((A, 3)), np.zeros((E, 8, 3)), np.zeros((E, 8, 3))
+        self._chk(self.L.cac_hip_download_forces(self.h, f.ctypes.data if A else None,
+                                                 nf.ctypes.data if E else None, gf.ctypes.data if E else None))
+        return f, nf, gf
 
     def full_sets(self):
         """Full lists as sorted rows (point, kind, owner, ucell, ix, iy, iz)."""

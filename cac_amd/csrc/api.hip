@@ -26,6 +26,7 @@ Ctx::~Ctx() {
   if (h_red) (void)hipHostFree(h_red);
   if (h_flags) (void)hipHostFree(h_flags);
   if (h_steps) (void)hipHostFree(const_cast<int*>(h_steps));
+  if (h_stage) (void)hipHostFree(h_stage);
   if (ev_check) (void)hipEventDestroy(ev_check);
   for (auto& row : ev)
     for (auto& e : row)
@@ -152,6 +153,17 @@ int require_model(Ctx& c) {
     return CAC_HIP_ESTATE;
   }
   return CAC_HIP_OK;
+}
+
+// pinned host staging for the per-step handover, grown on demand
+double* host_staging(Ctx& c, size_t n) {
+  if (n > c.h_stage_n) {
+    if (c.h_stage) CAC_HIP(hipHostFree(c.h_stage));
+    c.h_stage = nullptr;
+    CAC_HIP(hipHostMalloc((void**)&c.h_stage, sizeof(double) * n, hipHostMallocDefault));
+    c.h_stage_n = n;
+  }
+  return c.h_stage;
 }
 
 void compute_forces(Ctx& c, int eflag, int vflag) {
@@ -792,6 +804,51 @@ int cac_hip_download_elements(cac_hip_ctx* h, double* nodex, double* nodev, doub
       const size_t E8 = (size_t)NPE * E;
       std::vector<double> g(3 * E8);
       CAC_HIP(hipMemcpy(g.data(), c.gf.get(), sizeof(double) * 3 * E8, hipMemcpyDeviceToHost));
+      for (int e = 0; e < E; e++)
+        for (int q = 0; q < 8; q++)
+          for (int d = 0; d < 3; d++)
+            gaussf[((size_t)e * 8 + q) * 3 + d] = g[(size_t)d * E8 + (size_t)NPE * e + c.g2n[q]];
+    }
+    return CAC_HIP_OK;
+  });
+}
+
+int cac_hip_sync_positions(cac_hip_ctx* h, const double* x, const double* nodex) {
+  return guarded(h, [&](Ctx& c) -> int {
+    if (!c.model_set) return fail(h, CAC_HIP_ESTATE, "no atoms or elements uploaded");
+    if ((c.A && !x) || (c.E && !nodex)) return fail(h, CAC_HIP_EARG, "null positions");
+    const int A = c.A, P = c.P, E = c.E;
+    double* st = host_staging(c, (size_t)3 * P);
+    CAC_HIP(hipStreamSynchronize(c.stream));  // the staging buffer may still feed an earlier copy
+    for (int d = 0; d < 3; d++) {
+      double* o = st + (size_t)d * P;
+      for (int i = 0; i < A; i++) o[i] = x[3 * (size_t)i + d];
+      for (size_t q = 0; q < (size_t)NPE * E; q++) o[A + q] = nodex[3 * q + d];
+    }
+    CAC_HIP(hipMemcpyAsync(c.x.get(), st, sizeof(double) * 3 * P, hipMemcpyHostToDevice, c.stream));
+    c.x_dirty = true;
+    return CAC_HIP_OK;
+  });
+}
+
+int cac_hip_download_forces(cac_hip_ctx* h, double* f, double* nodef, double* gaussf) {
+  return guarded(h, [&](Ctx& c) -> int {
+    const int A = c.A, P = c.P, E = c.E;
+    const size_t E8 = (size_t)NPE * E;
+    double* st = host_staging(c, (size_t)3 * P + 3 * E8);
+    CAC_HIP(hipMemcpyAsync(st, c.f.get(), sizeof(double) * 3 * P, hipMemcpyDeviceToHost, c.stream));
+    if (gaussf && E)
+      CAC_HIP(hipMemcpyAsync(st + (size_t)3 * P, c.gf.get(), sizeof(double) * 3 * E8, hipMemcpyDeviceToHost,
+                             c.stream));
+    CAC_HIP(hipStreamSynchronize(c.stream));
+    if (f)
+      for (int i = 0; i < A; i++)
+        for (int d = 0; d < 3; d++) f[3 * (size_t)i + d] = st[(size_t)d * P + i];
+    if (nodef)
+      for (size_t q = 0; q < E8; q++)
+        for (int d = 0; d < 3; d++) nodef[3 * q + d] = st[(size_t)d * P + A + q];
+    if (gaussf) {
+      const double* g = st + (size_t)3 * P;
       for (int e = 0; e < E; e++)
         for (int q = 0; q < 8; q++)
           for (int d = 0; d < 3; d++)

@@ -249,6 +249,8 @@ struct Ctx {
   volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
   int* h_steps_dev = nullptr;       // device address of h_steps
   hipEvent_t ev_check = nullptr;    // recorded after the check of a speculative step
+  double* h_stage = nullptr;        // pinned staging of the per-step host handover
+  size_t h_stage_n = 0;
   DBuf<ErrorFlags> d_flags;
 
   // timing

@@ -108,6 +108,16 @@ int cac_hip_run(cac_hip_ctx *ctx, int nsteps, double dt, int thermo_every, doubl
  * compute_temp.cpp:73-157, compute_pressure.cpp:140-180) */
 int cac_hip_thermo(cac_hip_ctx *ctx, double *pe, double *ke, double *temp, double *press);
 
+/* ---- per-step host handover, for a caller that keeps the integrator on the host:
+ * Pair::compute reads atom->x and element->nodex and writes atom->f,
+ * element->gaussf and (after Element::compute_nodef) element->nodef
+ * (pair_eam.cpp:618-1032, element.cpp:2178-2246).  sync_positions copies the
+ * host positions in ([A][3], [E][1][8][3]) without touching the lists; the
+ * next cac_hip_neigh_decide / cac_hip_compute see them.  download_forces
+ * copies f, nodef and gaussf out (each NULL-able). */
+int cac_hip_sync_positions(cac_hip_ctx *ctx, const double *x, const double *nodex);
+int cac_hip_download_forces(cac_hip_ctx *ctx, double *f, double *nodef, double *gaussf);
+
 /* ---- host <-> device ---- */
 int cac_hip_download_atoms(cac_hip_ctx *ctx, double *x, double *v, double *f);
 int cac_hip_download_elements(cac_hip_ctx *ctx, double *nodex, double *nodev, double *nodef, double *gaussf);

@@ -5,6 +5,8 @@ energies, virials and densities within 1e-12 relative (max |a - b| over the
 array divided by max |b|); mixed precision forces within 1e-5 of FP64.
 """
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -187,3 +189,80 @@ def test_mixed_precision_force_error():
     e64, _ = g64.energy()
     e32, _ = g32.energy()
     assert abs(e32 - e64) < 1e-6 * abs(e64)
+
+
+# ------------------------------------------------------------------ host handover, closed forms, launch shapes
+
+def test_host_handover_matches_oracle():
+    """A caller that integrates on the host: positions in, decide, compute, forces out."""
+    deck = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=300.0, seed=12)
+    g = gpu_model(deck, "eam", 2.0)
+    g.setup()
+    rng = np.random.default_rng(4)
+    moved = dataclasses.replace(deck, x=deck.x + rng.normal(0, 0.03, deck.x.shape),
+                                nodex=deck.nodex + rng.normal(0, 0.03, deck.nodex.shape))
+    g.sync_positions(moved.x, moved.nodex)
+    if g.decide():
+        g.build_neighbor()
+    g.compute(1, 1)
+    f, nodef, gaussf = g.forces()
+    o = oracle_model(moved, "eam", 2.0)
+    o.setup()
+    _, _, fo = o.atoms()
+    eo = o.elements()
+    assert rel(f, fo) < 1e-12
+    assert rel(gaussf, eo["gaussf"]) < 1e-12
+    assert rel(nodef, eo["nodef"]) < 1e-12
+    assert abs(g.energy()[0] - o.energy()[0]) < 1e-12 * abs(o.energy()[0])
+
+
+@pytest.mark.parametrize("delay,expect", [(0, 6), (10, 5)])
+def test_rebuild_schedule_ballistic(delay, expect):
+    """One free atom at 0.07 A per step, skin 1.0 A: builds every 8 steps with
+    delay 0 (6 in 50), every 10 with delay 10, each of them dangerous (5)."""
+    g0 = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3) * 10.0 + 1.0
+    v = np.zeros_like(g0)
+    v[5] = (70.0, 0.0, 0.0)
+    deck = decks.Deck(units="metal", boxlo=np.zeros(3), boxhi=np.full(3, 40.0), masses=np.array([63.546]), x=g0,
+                      v=v, atype=np.ones(len(g0), dtype=np.int32))
+    g = gpu_model(deck, "eam", 1.0, delay=delay)
+    g.setup()
+    g.run(50, 0.001, thermo=False)
+    assert g.neigh_stats() == (expect, expect if delay else 0)
+    x, _, _ = g.atoms()
+    assert np.allclose(x[5], np.mod(g0[5] + v[5] * 0.05, 40.0), atol=1e-12)
+
+
+def test_periodic_wrap_moves_elements_rigidly():
+    deck = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.0, T=0.0, jitter_atoms=False)
+    V = np.array([900.0, -400.0, 250.0])
+    deck.v = np.tile(V, (deck.natoms, 1))
+    deck.nodev = np.tile(V, (deck.nelements, 8, 1))
+    g = gpu_model(deck, "eam", 2.0, delay=0)
+    g.setup()
+    g.run(60, 0.001, thermo=False)
+    L = deck.boxhi - deck.boxlo
+    x, _, f = g.atoms()
+    el = g.elements()
+    assert np.abs(f).max() < 1e-9 and np.abs(el["gaussf"]).max() < 1e-9
+    assert np.allclose(x, deck.boxlo + np.mod(deck.x + V * 0.06 - deck.boxlo, L), atol=1e-9)
+    nb = deck.nodex + V * 0.06
+    k = np.floor((nb.mean(axis=1) - deck.boxlo) / L)
+    assert np.allclose(el["nodex"], nb - k[:, None, :] * L, atol=1e-9)
+
+
+def test_launch_shape_does_not_change_results(monkeypatch):
+    """Workgroup size and entries in flight change the schedule only: the FP64
+    and mixed forces are bitwise the same for every pair-kernel shape."""
+    import cac_amd
+    deck = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=300.0, seed=13)
+    out = {}
+    for prec in (cac_amd.FP64, cac_amd.MIXED):
+        for shape in range(4):
+            monkeypatch.setenv("CAC_HIP_PAIR_SHAPE", str(shape))
+            g = gpu_model(deck, "eam", 2.0, precision=prec)
+            g.setup()
+            g.run(5, 0.001, thermo=False)
+            out[(prec, shape)] = np.concatenate([g.atoms()[2].ravel(), g.elements()["gaussf"].ravel()])
+        for shape in range(1, 4):
+            assert np.array_equal(out[(prec, 0)], out[(prec, shape)]), (prec, shape)
