@@ -48,6 +48,30 @@ def _compile(src: str) -> str:
     return obj
 
 
+HOST = os.path.join(HERE, "host")
+HOSTLIB = os.path.join(HERE, "libcachost.so")
+RUNNER = os.path.join(HERE, "cac_run")
+HOSTFLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", f"-I{os.path.join(ROOT, 'include')}"]
+
+
+def build_host(verbose: bool = False) -> None:
+    """The C++ host layer (Pair / Neighbor / FixNVE / Verlet mirrors, data-file
+    reader) as libcachost.so over libcachip.so, and the cac_run driver."""
+    steps = [
+        (["g++", *HOSTFLAGS, "-shared", "-o", HOSTLIB + ".tmp", os.path.join(HOST, "cac_host.cpp"), f"-L{HERE}",
+          "-lcachip", "-Wl,-rpath,$ORIGIN"], HOSTLIB),
+        (["g++", *HOSTFLAGS, "-o", RUNNER + ".tmp", os.path.join(HOST, "cac_run.cpp"), f"-L{HERE}", "-lcachost",
+          "-lcachip", "-Wl,-rpath,$ORIGIN"], RUNNER),
+    ]
+    for cmd, out in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed:\n{' '.join(cmd)}\n{r.stderr[-4000:]}")
+        os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {HOSTLIB} and {RUNNER}")
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
@@ -61,6 +85,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_host(verbose)
     return LIB
 
 
