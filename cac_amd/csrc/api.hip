@@ -209,8 +209,7 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     // step words of the speculative step: device copy for the gated kernels,
     // host-mapped copy for the host
     c.d_steps.reserve(2);
-    c.live_gen.reserve(1);
-    CAC_HIP(hipMemsetAsync(c.live_gen.get(), 0, sizeof(int), c.stream));
+
     CAC_HIP(hipMemsetAsync(c.d_steps.get(), 0xFF, sizeof(int) * 2, c.stream));
     int* hs = nullptr;
     CAC_HIP(hipHostMalloc((void**)&hs, sizeof(int) * 4, hipHostMallocMapped | hipHostMallocCoherent));
@@ -738,25 +737,23 @@ int cac_hip_run(cac_hip_ctx* h, int nsteps, double dt, int thermo_every, double*
         pair_passes(c, ev, ev, Gate());
         step_end(c, dt, Gate());
       } else {
-        // queue the step as if no rebuild were due; its kernels return at once
-        // if the check in step_begin asks for one, and the host, which waits
-        // only for that check, then rebuilds and runs the step again
+        // queue the site refresh as if no rebuild were due (it returns at once
+        // if the check in step_begin asks for one, and refreshes every site if
+        // the check asks for a re-prune); the host waits for the check while
+        // the refresh runs, then queues the rest of the step
         const Gate g{c.d_steps.get(), N};
         step_begin(c, dt, N, due);
         CAC_HIP(hipEventRecord(c.ev_check, c.stream));
         site_refresh(c, false, g);
-        build_inner(c, g);
-        pair_passes(c, ev, ev, g);
-        step_end(c, dt, g);
         CAC_HIP(hipEventSynchronize(c.ev_check));
         if (c.h_steps[0] == N) {
           if (c.ago == std::max(c.every, c.delay)) c.ndanger++;
           build_neighbor(c);
-          pair_passes(c, ev, ev, Gate());
-          step_end(c, dt, Gate());
         } else if (c.h_steps[1] == N) {
-          c.inner_builds++;
+          build_inner(c);
         }
+        pair_passes(c, ev, ev, Gate());
+        step_end(c, dt, Gate());
       }
       if (ev) pair_sums(c, ev, ev);
       if (ev) record();
@@ -919,13 +916,9 @@ int cac_hip_inner_stats(cac_hip_ctx* h, long* builds, double* mean_entries, doub
   return guarded(h, [&](Ctx& c) -> int {
     if (live_sites) {
       int n = 0;
-      if (c.built && c.S && c.inner_mode >= 0) {
-        std::vector<int> st(c.S);
-        int gen = 0;
+      if (c.built && c.inner_mode >= 0) {
         CAC_HIP(hipStreamSynchronize(c.stream));
-        CAC_HIP(hipMemcpy(st.data(), c.live.get(), sizeof(int) * c.S, hipMemcpyDeviceToHost));
-        CAC_HIP(hipMemcpy(&gen, c.live_gen.get(), sizeof(int), hipMemcpyDeviceToHost));
-        for (int v : st) n += v == gen;
+        CAC_HIP(hipMemcpy(&n, c.d_nlive.get(), sizeof(int), hipMemcpyDeviceToHost));
       }
       *live_sites = n;
     }

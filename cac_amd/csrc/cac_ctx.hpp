@@ -52,15 +52,30 @@ __host__ __device__ inline int cell_iz(int p) { return (p >> (CELL_BITS_X + CELL
 // are bitwise those of the table.
 struct UcellGrid {
   double h[3];
-  int ny, nz;
+  int ny, nz, nucell;
+  double inv_nucell;      // 1 / nucell
+  float inv_nyz, inv_nz;  // 1 / (ny nz), 1 / nz
 };
+
+// a / b for 0 <= a < 2^31, b > 0, from an estimate off by at most one
+__device__ __forceinline__ int div_fix(int a, int b, int est) {
+  est -= est * b > a ? 1 : 0;
+  est += (est + 1) * b <= a ? 1 : 0;
+  return est;
+}
+
+// element e and unit cell u of the interpolated-atom descriptor q = e nucell + u
+__device__ __forceinline__ void split_ucell(const UcellGrid& ug, int q, int& e, int& u) {
+  e = div_fix(q, ug.nucell, (int)((double)q * ug.inv_nucell));
+  u = q - e * ug.nucell;
+}
 
 __device__ __forceinline__ void hex_shape_at(const UcellGrid& ug, int u, double* N) {
 #pragma clang fp contract(off)
   const int nyz = ug.ny * ug.nz;
-  const int i = u / nyz;
+  const int i = div_fix(u, nyz, (int)((float)u * ug.inv_nyz));
   const int r = u - i * nyz;
-  const int j = r / ug.nz;
+  const int j = div_fix(r, ug.nz, (int)((float)r * ug.inv_nz));
   const int k = r - j * ug.nz;
   const double px = ug.h[0] * i - 1.0, py = ug.h[1] * j - 1.0, pz = ug.h[2] * k - 1.0;
   N[0] = 0.125 * (1 - px) * (1 - py) * (1 - pz);
@@ -244,8 +259,10 @@ struct Ctx {
   double* h_red = nullptr;  // pinned host scratch for reductions
   ErrorFlags* h_flags = nullptr;  // pinned mirror of the device flags
   DBuf<int> d_steps;              // [rebuild step, prune step] written by the displacement check
-  DBuf<int> live, live_gen;       // per-site stamp of the last prune that kept it, and that prune's stamp
-  int live_serial = 0;            // stamps of ungated prunes: -1, -2, ...
+  // live sites: the sites the inner lists read, compacted after every prune
+  // into dense arrays in site (cell) order; the inner lists and the rows index them
+  DBuf<int> live_flag, live_idx, lsdesc, lsimg, lscell, row_live, d_nlive;
+  DBuf<double4> site_live;        // FP64 records (x, y, z, F') of the live sites
   volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
   int* h_steps_dev = nullptr;       // device address of h_steps
   hipEvent_t ev_check = nullptr;    // recorded after the check of a speculative step
@@ -268,7 +285,8 @@ void launch_pbc(Ctx& c);
 void check_deformation(Ctx& c, bool first);
 void build_neighbor(Ctx& c);  // pbc + sites + cells + full lists
 int check_distance(Ctx& c, bool outer, int* inner);  // 1 when any point moved more than skin/2
-void build_inner(Ctx& c, Gate g = Gate());  // prune the outer list to rc + delta for the force passes
+void build_inner(Ctx& c);  // prune the outer list to rc + delta and compact the live sites
+void compact_live(Ctx& c);  // dense live-site records from the full ones (after a prune)
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
                   std::vector<int>& ucell, std::vector<int>& img);
 void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total);

@@ -636,9 +636,7 @@ void build_neighbor(Ctx& c) {
     c.S = S2;
     c.nvirt = S2 - P;
   }
-  // no site is live for the force passes until the first prune of this layout
-  c.live.reserve(c.S);
-  CAC_HIP(hipMemsetAsync(c.live.get(), 0x80, sizeof(int) * c.S, c.stream));
+
   c.entries = -1;  // stored entries, counted on request
   c.built = true;
   c.inner_mode = -1;
@@ -706,20 +704,19 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
                                                const int* __restrict__ nnum, const unsigned* __restrict__ nlist,
                                                int R, int cap, Grid g, double cutsq_in, int* __restrict__ innum,
                                                unsigned* __restrict__ ilist, const double* __restrict__ x,
-                                               double* __restrict__ xhold_in, int* __restrict__ live, int gen,
-                                               int* __restrict__ live_gen, Gate gate) {
-  if (!gate.prune()) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *live_gen = gen;
+                                               double* __restrict__ xhold_in, int* __restrict__ live) {
   constexpr int PU = 4;  // 16-entry groups in flight per lane group
   const int lane = threadIdx.x & 15;
   const int gbase = (threadIdx.x & 63) & ~15;
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
-  if (r >= R) return;
+  // grid-stride over rows, so a step that does not re-prune launches few workgroups
+  const long rstride = ((long)gridDim.x * blockDim.x) >> 4;
+  for (long rl = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4; rl < R; rl += rstride) {
+  const int r = (int)rl;
   if (lane < 3) {
     const size_t i = (size_t)lane * R + row_point[r];
     xhold_in[i] = x[i];
   }
-  if (lane == 3) live[row_site[r]] = gen;
+  if (lane == 3) live[row_site[r]] = 1;
   const int n = nnum[r];
   const int k = row_site[r];
   const double4 pk = site[k];
@@ -764,15 +761,39 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
       const int rank = __popc(gm & ((1u << lane) - 1u));
       if (keep) {
         out[count + rank] = eout;
-        live[ent[u] & SITE_MASK] = gen;
+        live[ent[u] & SITE_MASK] = 1;
       }
       count += __popc(gm);
     }
   }
   if (lane == 0) innum[r] = count;
+  }
 }
 
-void build_inner(Ctx& c, Gate gate) {
+// live-site count, dense index of every row's site, and the inner-list
+// entries re-pointed at the dense records (16 lanes per row)
+__global__ void k_live_count(const int* __restrict__ flag, const int* __restrict__ idx, int S, int* __restrict__ n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *n = S ? idx[S - 1] + flag[S - 1] : 0;
+}
+
+__global__ __launch_bounds__(256) void k_live_remap(const int* __restrict__ idx, const int* __restrict__ row_site,
+                                                    int* __restrict__ row_live, const int* __restrict__ innum,
+                                                    unsigned* __restrict__ ilist, int R, int cap, unsigned mask) {
+  const int lane = threadIdx.x & 15;
+  const long rstride = ((long)gridDim.x * blockDim.x) >> 4;
+  for (long rl = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4; rl < R; rl += rstride) {
+    const int r = (int)rl;
+    if (lane == 0) row_live[r] = idx[row_site[r]];
+    unsigned* row = ilist + (size_t)r * cap;
+    const int n = innum[r];
+    for (int i = lane; i < n; i += 16) {
+      const unsigned e = row[i];
+      row[i] = (unsigned)idx[e & mask] | (e & ~mask);
+    }
+  }
+}
+
+void build_inner(Ctx& c) {
   ScopedTimer tm(c, "inner_build");
   const bool mixed = c.precision == 1;
   if (mixed && (long)c.S >= (1L << MIX_SHIFT)) throw HipError{hipErrorInvalidValue, "more than 2^25 sites for the mixed list"};
@@ -782,24 +803,29 @@ void build_inner(Ctx& c, Gate gate) {
   c.icap = c.cap;
   c.innum.reserve(c.P);
   c.ilist.reserve((size_t)c.icap * c.P);
-  const int blocks = grid_for(16L * c.P, 256);
+  c.live_flag.reserve(c.S);
+  c.live_idx.reserve(c.S);
+  c.row_live.reserve(c.P);
+  c.d_nlive.reserve(1);
+  CAC_HIP(hipMemsetAsync(c.live_flag.get(), 0, sizeof(int) * c.S, c.stream));
+  const int blocks = (int)std::min<long>(grid_for(16L * c.P, 256), 8L * c.ncu);
   const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
   unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());
-  // stamp of the sites the pruned lists reference: the step number for a
-  // gated prune, a fresh negative serial otherwise, so stamps never repeat
-  const int gen = gate.step >= 0 ? gate.step : --c.live_serial;
   if (mixed)
     k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.row_point.get(),
                                                  c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
-                                                 c.x.get(), c.xhold_in.get(), c.live.get(), gen, c.live_gen.get(),
-                                                 gate);
+                                                 c.x.get(), c.xhold_in.get(), c.live_flag.get());
   else
     k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.row_point.get(),
                                                   c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
-                                                  c.x.get(), c.xhold_in.get(), c.live.get(), gen, c.live_gen.get(),
-                                                  gate);
+                                                  c.x.get(), c.xhold_in.get(), c.live_flag.get());
+  scan_exclusive(c, c.live_flag.get(), c.live_idx.get(), c.S, nullptr);
+  k_live_count<<<1, 64, 0, c.stream>>>(c.live_flag.get(), c.live_idx.get(), c.S, c.d_nlive.get());
+  compact_live(c);
+  k_live_remap<<<blocks, 256, 0, c.stream>>>(c.live_idx.get(), c.row_site.get(), c.row_live.get(), c.innum.get(), il,
+                                             c.P, c.icap, mixed ? (1u << MIX_SHIFT) - 1u : SITE_MASK);
   c.inner_mode = c.precision;
-  if (gate.step < 0) c.inner_builds++;  // a gated prune is counted once the host knows it ran
+  c.inner_builds++;
 }
 
 // ------------------------------------------------------------------ export for parity checks

@@ -137,47 +137,97 @@ struct Frame {
 // the step re-prunes the inner list
 // Only the sites the inner lists reference (stamped by the last prune) are
 // refreshed, except on a step that builds or prunes lists, which reads all.
-template <int W64, bool W32>
-__global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, UcellGrid ug,
-                                                 const int* __restrict__ sdesc, const int* __restrict__ simg,
-                                                 const int* __restrict__ scell, int S, int A, int P, int nucell,
-                                                 Frame fr, double4* __restrict__ site, float4* __restrict__ site32,
-                                                 const int* __restrict__ live, const int* __restrict__ live_gen,
-                                                 int all, Gate g) {
+// Two sites per thread (k and k + 256 of the block's range), every
+// descriptor load issued before any is used, so that the two sites' memory
+// round trips overlap.
+//
+// Full mode (a neighbor build, or a step that re-prunes the inner list)
+// refreshes every site's FP64 record, which the list build and the prune
+// read; the dense live records then come from them (compact_live).
+// Otherwise only the live sites are refreshed, straight into their dense
+// records: FP32 cell-relative in the mixed path, FP64 otherwise.
+constexpr int RSPT = 2;
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, UcellGrid ug, Frame fr, int S, int A,
+                                                 int P, const int* __restrict__ sdesc, const int* __restrict__ simg,
+                                                 double4* __restrict__ site, const int* __restrict__ nlive,
+                                                 const int* __restrict__ lsdesc, const int* __restrict__ lsimg,
+                                                 const int* __restrict__ lscell, float4* __restrict__ site32,
+                                                 double4* __restrict__ site_live, int all, Gate g) {
 #pragma clang fp contract(off)
   if (g.skip()) return;
-  const bool pruning = g.step >= 0 && g.prune();
-  const bool w64 = W64 == 1 || (W64 == 2 && pruning);
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S) return;
-  if (!all && !pruning && live[k] != *live_gen) return;
-  const int d = sdesc[k];
-  double y[3];
-  if (d >= 0) {
-    for (int c = 0; c < 3; c++) y[c] = x[(size_t)c * P + d];
-  } else {
-    const int q = -d - 1;
-    const int e = q / nucell, u = q - e * nucell;
-    double N[NPE];
-    hex_shape_at(ug, u, N);
-    const size_t base = (size_t)A + (size_t)NPE * e;
-    for (int c = 0; c < 3; c++) {
-      const double* nx = x + (size_t)c * P + base;
-      double s = 0.0;
-      for (int n = 0; n < NPE; n++) s += N[n] * nx[n];
-      y[c] = s;
+  const bool full = all || (g.step >= 0 && g.prune());
+  const int n = full ? S : *nlive;
+  const int* dsc = full ? sdesc : lsdesc;
+  const int* img = full ? simg : lsimg;
+  const int k0 = blockIdx.x * blockDim.x * RSPT + threadIdx.x;
+  int kk[RSPT], d[RSPT], im[RSPT], cc[RSPT];
+#pragma unroll
+  for (int t = 0; t < RSPT; t++) {
+    kk[t] = k0 + t * blockDim.x;
+    const int kc = kk[t] < n ? kk[t] : 0;
+    d[t] = dsc[kc];
+    im[t] = img[kc];
+    cc[t] = MIXED && !full ? lscell[kc] : 0;
+  }
+#pragma unroll
+  for (int t = 0; t < RSPT; t++) {
+    if (kk[t] >= n) continue;
+    double y[3];
+    if (d[t] >= 0) {
+      for (int c = 0; c < 3; c++) y[c] = x[(size_t)c * P + d[t]];
+    } else {
+      int e, u;
+      split_ucell(ug, -d[t] - 1, e, u);
+      double N[NPE];
+      hex_shape_at(ug, u, N);
+      const size_t base = (size_t)A + (size_t)NPE * e;
+      for (int c = 0; c < 3; c++) {
+        const double* nx = x + (size_t)c * P + base;
+        double s = 0.0;
+        for (int q = 0; q < NPE; q++) s += N[q] * nx[q];
+        y[c] = s;
+      }
     }
+    double sh[3];
+    img_shift((unsigned)im[t], fr.L, sh[0], sh[1], sh[2]);
+    for (int c = 0; c < 3; c++) y[c] = y[c] + sh[c];
+    const int k = kk[t];
+    if (full)
+      site[k] = make_double4(y[0], y[1], y[2], 0.0);
+    else if (MIXED)
+      site32[k] = make_float4((float)(y[0] - (fr.lo[0] + cell_ix(cc[t]) * fr.cs[0])),
+                              (float)(y[1] - (fr.lo[1] + cell_iy(cc[t]) * fr.cs[1])),
+                              (float)(y[2] - (fr.lo[2] + cell_iz(cc[t]) * fr.cs[2])), 0.0f);
+    else
+      site_live[k] = make_double4(y[0], y[1], y[2], 0.0);
   }
-  double sh[3];
-  img_shift((unsigned)simg[k], fr.L, sh[0], sh[1], sh[2]);
-  for (int c = 0; c < 3; c++) y[c] = y[c] + sh[c];
-  if (w64) site[k] = make_double4(y[0], y[1], y[2], 0.0);
-  if (W32) {
-    const int cc = scell[k];
-    site32[k] = make_float4((float)(y[0] - (fr.lo[0] + cell_ix(cc) * fr.cs[0])),
-                            (float)(y[1] - (fr.lo[1] + cell_iy(cc) * fr.cs[1])),
-                            (float)(y[2] - (fr.lo[2] + cell_iz(cc) * fr.cs[2])), 0.0f);
-  }
+}
+
+// dense live records from the full ones, after a prune marked the live sites
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_live_compact(const int* __restrict__ flag, const int* __restrict__ idx, int S,
+                                                      Frame fr, const int* __restrict__ sdesc,
+                                                      const int* __restrict__ simg, const int* __restrict__ scell,
+                                                      const double4* __restrict__ site, int* __restrict__ lsdesc,
+                                                      int* __restrict__ lsimg, int* __restrict__ lscell,
+                                                      float4* __restrict__ site32, double4* __restrict__ site_live) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S || !flag[k]) return;
+  const int i = idx[k];
+  const int cc = scell[k];
+  lsdesc[i] = sdesc[k];
+  lsimg[i] = simg[k];
+  lscell[i] = cc;
+  const double4 y = site[k];
+  if (MIXED)
+    site32[i] = make_float4((float)(y.x - (fr.lo[0] + cell_ix(cc) * fr.cs[0])),
+                            (float)(y.y - (fr.lo[1] + cell_iy(cc) * fr.cs[1])),
+                            (float)(y.z - (fr.lo[2] + cell_iz(cc) * fr.cs[2])), 0.0f);
+  else
+    site_live[i] = make_double4(y.x, y.y, y.z, 0.0);
 }
 
 static UcellGrid ucell_grid(const Ctx& c) {
@@ -185,6 +235,10 @@ static UcellGrid ucell_grid(const Ctx& c) {
   for (int d = 0; d < 3; d++) ug.h[d] = c.ncell[d] > 1 ? 2.0 / (c.ncell[d] - 1) : 0.0;
   ug.ny = c.ncell[1] > 0 ? c.ncell[1] : 1;
   ug.nz = c.ncell[2] > 0 ? c.ncell[2] : 1;
+  ug.nucell = c.nucell > 0 ? c.nucell : 1;
+  ug.inv_nucell = 1.0 / ug.nucell;
+  ug.inv_nyz = 1.0f / (float)(ug.ny * ug.nz);
+  ug.inv_nz = 1.0f / (float)ug.nz;
   return ug;
 }
 
@@ -198,31 +252,38 @@ static Frame frame_of(const Ctx& c) {
   return fr;
 }
 
+static void reserve_live(Ctx& c) {
+  c.lsdesc.reserve(c.S);
+  c.lsimg.reserve(c.S);
+  c.lscell.reserve(c.S);
+  if (c.precision == 1)
+    c.site32.reserve(c.S);
+  else
+    c.site_live.reserve(c.S);
+}
+
 void site_refresh(Ctx& c, bool full, Gate g) {
   ScopedTimer tm(c, "site_refresh");
   c.x_dirty = false;
   const bool mixed = c.precision == 1;
-  if (mixed) c.site32.reserve(c.S);
+  reserve_live(c);
+  c.d_nlive.reserve(1);
   const Frame fr = frame_of(c);
   const UcellGrid ug = ucell_grid(c);
-  const int nuc = c.nucell > 0 ? c.nucell : 1;
-  const int grid = grid_for(c.S, 256);
-  float4* s32 = c.site32.get();
-  const int all = full ? 1 : 0;
-  const int* lv = c.live.get();
-  const int* lg = c.live_gen.get();
-  if (!mixed)
-    k_refresh<1, false><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(), c.scell.get(), c.S,
-                                                    c.A, c.P, nuc, fr, c.site.get(), nullptr, lv, lg, all, g);
-  else if (full)
-    k_refresh<1, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(), c.scell.get(), c.S,
-                                                   c.A, c.P, nuc, fr, c.site.get(), s32, lv, lg, all, g);
-  else if (g.step >= 0)
-    k_refresh<2, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(), c.scell.get(), c.S,
-                                                   c.A, c.P, nuc, fr, c.site.get(), s32, lv, lg, all, g);
-  else
-    k_refresh<0, true><<<grid, 256, 0, c.stream>>>(c.x.get(), ug, c.sdesc.get(), c.simg.get(), c.scell.get(), c.S,
-                                                   c.A, c.P, nuc, fr, nullptr, s32, lv, lg, all, g);
+  const int grid = grid_for(c.S, 256 * RSPT);
+  auto kr = mixed ? k_refresh<true> : k_refresh<false>;
+  kr<<<grid, 256, 0, c.stream>>>(c.x.get(), ug, fr, c.S, c.A, c.P, c.sdesc.get(), c.simg.get(), c.site.get(),
+                                 c.d_nlive.get(), c.lsdesc.get(), c.lsimg.get(), c.lscell.get(), c.site32.get(),
+                                 c.site_live.get(), full ? 1 : 0, g);
+}
+
+void compact_live(Ctx& c) {
+  reserve_live(c);
+  const Frame fr = frame_of(c);
+  auto kc = c.precision == 1 ? k_live_compact<true> : k_live_compact<false>;
+  kc<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.live_flag.get(), c.live_idx.get(), c.S, fr, c.sdesc.get(),
+                                               c.simg.get(), c.scell.get(), c.site.get(), c.lsdesc.get(),
+                                               c.lsimg.get(), c.lscell.get(), c.site32.get(), c.site_live.get());
 }
 
 // ------------------------------------------------------------------ spline helpers
@@ -279,7 +340,10 @@ __device__ __forceinline__ void stage_table_strided(V* lds, const V* __restrict_
 // ------------------------------------------------------------------ row walker
 // Lane l of a row's TPR lanes visits entries l, l + TPR, ...; U entries at a
 // time: U list loads, then U site gathers, then the U pair terms, so U
-// independent memory chains are in flight per lane instead of one.
+// independent memory chains are in flight per lane instead of one.  The pair
+// terms take a validity flag instead of a branch (an entry past the row's end
+// or outside the cutoff contributes an exact zero), so the U terms form one
+// straight-line block the compiler can interleave.
 
 constexpr int UNROLL = 4;
 
@@ -297,8 +361,7 @@ __device__ __forceinline__ void walk_row(const unsigned* __restrict__ row, int n
 #pragma unroll
     for (int u = 0; u < U; u++) sj[u] = fetch(e[u]);
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      if (j0 + u * TPR < n) body(e[u], sj[u]);
+    for (int u = 0; u < U; u++) body(e[u], sj[u], j0 + u * TPR < n);
   }
 }
 
@@ -335,36 +398,34 @@ __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __re
         const float4 pi = a.site32[k];
         walk_row<TPR, U, float4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
-            [&](unsigned ent, const float4& sj) {
+            [&](unsigned ent, const float4& sj, bool valid) {
               const float4 o = offs[ent >> MIX_SHIFT];
               const float dx = (pi.x - sj.x) - o.x;
               const float dy = (pi.y - sj.y) - o.y;
               const float dz = (pi.z - sj.z) - o.z;
               const float rsq = dx * dx + dy * dy + dz * dz;
-              if (rsq < cut32) {
-                float p;
-                const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, nr, p);
-                const float4 c = lds32[m];
-                acc += (double)(((c.x * p + c.y) * p + c.z) * p + c.w);
-              }
+              const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
+              float p;
+              const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, nr, p);
+              const float4 c = lds32[m];
+              acc += (double)(okf * (((c.x * p + c.y) * p + c.z) * p + c.w));
             });
       } else {
         const double4 pi = a.site[k];
         walk_row<TPR, U, double4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
-            [&](unsigned ent, const double4& sj) {
+            [&](unsigned ent, const double4& sj, bool valid) {
               double shx, shy, shz;
               img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
               const double delx = pi.x - (sj.x + shx);
               const double dely = pi.y - (sj.y + shy);
               const double delz = pi.z - (sj.z + shz);
               const double rsq = delx * delx + dely * dely + delz * delz;
-              if (rsq < a.cutsq) {
-                double p;
-                const int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
-                const double4 c = lds64[m];
-                acc += ((c.x * p + c.y) * p + c.z) * p + c.w;
-              }
+              const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
+              double p;
+              const int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
+              const double4 c = lds64[m];
+              acc += okf * (((c.x * p + c.y) * p + c.z) * p + c.w);
             });
       }
     }
@@ -387,38 +448,43 @@ __device__ __forceinline__ double fprime(const double* __restrict__ frho, double
 
 // F'(rho) of every live site (compute_fp, pair_eam.cpp:1160-1169): a point's
 // own density, or the shape-function interpolation of its element's nodal
-// densities for an interpolated atom (pair_eam.cpp:700-760); one thread per
-// site in site order.  Only the F' word of each record is written.
+// densities for an interpolated atom (pair_eam.cpp:700-760); two live sites
+// per thread in dense order.  Only the F' word of each record is written.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, UcellGrid ug,
-                                               const int* __restrict__ sdesc, int S, int A, int nucell,
+                                               const int* __restrict__ lsdesc, const int* __restrict__ nlive, int A,
                                                const double* __restrict__ frho, double rdrho, int nrho,
-                                               double4* __restrict__ site, float4* __restrict__ site32,
-                                               const int* __restrict__ live, const int* __restrict__ live_gen,
-                                               Gate g) {
+                                               double4* __restrict__ site_live, float4* __restrict__ site32) {
 #pragma clang fp contract(off)
-  if (g.skip()) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S) return;
-  if (live[k] != *live_gen) return;  // no inner list reads this site
-  const int d = sdesc[k];
-  double rj;
-  if (d >= 0) {
-    rj = rho[d];
-  } else {
-    const int q = -d - 1;
-    const int e = q / nucell, u = q - e * nucell;
-    double N[NPE];
-    hex_shape_at(ug, u, N);
-    const double* nr = rho + (size_t)A + (size_t)NPE * e;
-    rj = 0.0;
-    for (int n = 0; n < NPE; n++) rj += N[n] * nr[n];
+  const int n = *nlive;
+  const int k0 = blockIdx.x * blockDim.x * RSPT + threadIdx.x;
+  int kk[RSPT], d[RSPT];
+#pragma unroll
+  for (int t = 0; t < RSPT; t++) {
+    kk[t] = k0 + t * blockDim.x;
+    d[t] = lsdesc[kk[t] < n ? kk[t] : 0];
   }
-  const double fp = fprime(frho, rj, rdrho, nrho);
-  if (MIXED)
-    reinterpret_cast<float*>(site32 + k)[3] = (float)fp;
-  else
-    reinterpret_cast<double*>(site + k)[3] = fp;
+#pragma unroll
+  for (int t = 0; t < RSPT; t++) {
+    if (kk[t] >= n) continue;
+    double rj;
+    if (d[t] >= 0) {
+      rj = rho[d[t]];
+    } else {
+      int e, u;
+      split_ucell(ug, -d[t] - 1, e, u);
+      double N[NPE];
+      hex_shape_at(ug, u, N);
+      const double* nr = rho + (size_t)A + (size_t)NPE * e;
+      rj = 0.0;
+      for (int q = 0; q < NPE; q++) rj += N[q] * nr[q];
+    }
+    const double fp = fprime(frho, rj, rdrho, nrho);
+    if (MIXED)
+      reinterpret_cast<float*>(site32 + kk[t])[3] = (float)fp;
+    else
+      reinterpret_cast<double*>(site_live + kk[t])[3] = fp;
+  }
 }
 
 // ------------------------------------------------------------------ EAM force (pair_eam.cpp:853-1027)
@@ -466,76 +532,76 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
         const float cut32 = (float)a.cutsq;
         walk_row<TPR, U, float4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
-            [&](unsigned ent, const float4& sj) {
+            [&](unsigned ent, const float4& sj, bool valid) {
               const float4 o = offs[ent >> MIX_SHIFT];
               const float dx = (pi.x - sj.x) - o.x;
               const float dy = (pi.y - sj.y) - o.y;
               const float dz = (pi.z - sj.z) - o.z;
-              const float rsq = dx * dx + dy * dy + dz * dz;
-              if (rsq < cut32) {
-                const float recip = __builtin_amdgcn_rsqf(rsq);
-                const float rr = rsq * recip;
-                float p;
-                const int m = knot_r32(rr, rdr32, q.nr, p);
-                const float4 cz = lds32[2 * m];      // z2r value c3..c6
-                const float4 cd = lds32[2 * m + 1];  // rhor derivative c0..c2, z2r c2
-                const float rhop = (cd.x * p + cd.y) * p + cd.z;
-                const float z2p = ((cz.x * k3) * p + cz.y * k2) * p + cd.w;
-                const float z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
-                const float phi = z2 * recip;
-                const float phip = (z2p - phi) * recip;
-                const float fpair = -((pi.w + sj.w) * rhop + phip) * recip;
-                fx += (double)(dx * fpair);
-                fy += (double)(dy * fpair);
-                fz += (double)(dz * fpair);
-                if (EV) {
-                  ephi += (double)phi;
-                  v0 += (double)(dx * dx * fpair);
-                  v1 += (double)(dy * dy * fpair);
-                  v2 += (double)(dz * dz * fpair);
-                  v3 += (double)(dx * dy * fpair);
-                  v4 += (double)(dx * dz * fpair);
-                  v5 += (double)(dy * dz * fpair);
-                }
+              // rsq is floored so that a masked entry (the row's own site past
+              // the row's end) stays finite and contributes 0 * finite
+              const float rsq = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+              const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
+              const float recip = __builtin_amdgcn_rsqf(rsq);
+              const float rr = rsq * recip;
+              float p;
+              const int m = knot_r32(rr, rdr32, q.nr, p);
+              const float4 cz = lds32[2 * m];      // z2r value c3..c6
+              const float4 cd = lds32[2 * m + 1];  // rhor derivative c0..c2, z2r c2
+              const float rhop = (cd.x * p + cd.y) * p + cd.z;
+              const float z2p = ((cz.x * k3) * p + cz.y * k2) * p + cd.w;
+              const float z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
+              const float phi = z2 * recip;
+              const float phip = (z2p - phi) * recip;
+              const float fpair = okf * (-((pi.w + sj.w) * rhop + phip) * recip);
+              fx += (double)(dx * fpair);
+              fy += (double)(dy * fpair);
+              fz += (double)(dz * fpair);
+              if (EV) {
+                ephi += (double)(okf * phi);
+                v0 += (double)(dx * dx * fpair);
+                v1 += (double)(dy * dy * fpair);
+                v2 += (double)(dz * dz * fpair);
+                v3 += (double)(dx * dy * fpair);
+                v4 += (double)(dx * dz * fpair);
+                v5 += (double)(dy * dz * fpair);
               }
             });
       } else {
         const double4 pi = a.site[k];
         walk_row<TPR, U, double4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
-            [&](unsigned ent, const double4& sj) {
+            [&](unsigned ent, const double4& sj, bool valid) {
               double shx, shy, shz;
               img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
               const double delx = pi.x - (sj.x + shx);
               const double dely = pi.y - (sj.y + shy);
               const double delz = pi.z - (sj.z + shz);
-              const double rsq = delx * delx + dely * dely + delz * delz;
-              if (rsq < a.cutsq) {
-                const double rr = sqrt(rsq);
-                double p;
-                const int m = knot_r<double>(rr, q.rdr, q.nr, p);
-                const double4 cr = lds64[2 * m];
-                const double4 cz = lds64[2 * m + 1];
-                const double rhop = ((3.0 * cr.x * p + 2.0 * cr.y) * p + cr.z) * q.rdr;
-                const double z2p = ((3.0 * cz.x * p + 2.0 * cz.y) * p + cz.z) * q.rdr;
-                const double z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
-                const double recip = 1.0 / rr;
-                const double phi = z2 * recip;
-                const double phip = z2p * recip - phi * recip;
-                const double psip = pi.w * rhop + sj.w * rhop + phip;
-                const double fpair = -1.0 * psip * recip;
-                fx += delx * fpair;
-                fy += dely * fpair;
-                fz += delz * fpair;
-                if (EV) {
-                  ephi += phi;
-                  v0 += delx * delx * fpair;
-                  v1 += dely * dely * fpair;
-                  v2 += delz * delz * fpair;
-                  v3 += delx * dely * fpair;
-                  v4 += delx * delz * fpair;
-                  v5 += dely * delz * fpair;
-                }
+              const double rsq = fmax(delx * delx + dely * dely + delz * delz, 1e-200);
+              const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
+              const double rr = sqrt(rsq);
+              double p;
+              const int m = knot_r<double>(rr, q.rdr, q.nr, p);
+              const double4 cr = lds64[2 * m];
+              const double4 cz = lds64[2 * m + 1];
+              const double rhop = ((3.0 * cr.x * p + 2.0 * cr.y) * p + cr.z) * q.rdr;
+              const double z2p = ((3.0 * cz.x * p + 2.0 * cz.y) * p + cz.z) * q.rdr;
+              const double z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
+              const double recip = 1.0 / rr;
+              const double phi = z2 * recip;
+              const double phip = z2p * recip - phi * recip;
+              const double psip = pi.w * rhop + sj.w * rhop + phip;
+              const double fpair = okf * (-1.0 * psip * recip);
+              fx += delx * fpair;
+              fy += dely * fpair;
+              fz += delz * fpair;
+              if (EV) {
+                ephi += okf * phi;
+                v0 += delx * delx * fpair;
+                v1 += dely * dely * fpair;
+                v2 += delz * delz * fpair;
+                v3 += delx * dely * fpair;
+                v4 += delx * delz * fpair;
+                v5 += dely * delz * fpair;
               }
             });
       }
@@ -616,57 +682,55 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
         const float cut32 = (float)a.cutsq;
         walk_row<TPR, UNROLL, float4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
-            [&](unsigned ent, const float4& sj) {
+            [&](unsigned ent, const float4& sj, bool valid) {
               float dx, dy, dz;
               mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
-              const float rsq = dx * dx + dy * dy + dz * dz;
-              if (rsq < cut32) {
-                const float r2inv = 1.0f / rsq;
-                const float r6inv = r2inv * r2inv * r2inv;
-                const float forcelj = r6inv * (l1 * r6inv - l2);
-                const float fpair = forcelj * r2inv;
-                fx += (double)(dx * fpair);
-                fy += (double)(dy * fpair);
-                fz += (double)(dz * fpair);
-                if (EV) {
-                  ev += (double)(r6inv * (l3 * r6inv - l4));
-                  v0 += (double)(dx * dx * fpair);
-                  v1 += (double)(dy * dy * fpair);
-                  v2 += (double)(dz * dz * fpair);
-                  v3 += (double)(dx * dy * fpair);
-                  v4 += (double)(dx * dz * fpair);
-                  v5 += (double)(dy * dz * fpair);
-                }
+              const float rsq = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+              const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
+              const float r2inv = __builtin_amdgcn_rcpf(rsq);
+              const float r6inv = r2inv * r2inv * r2inv;
+              const float forcelj = r6inv * (l1 * r6inv - l2);
+              const float fpair = okf * (forcelj * r2inv);
+              fx += (double)(dx * fpair);
+              fy += (double)(dy * fpair);
+              fz += (double)(dz * fpair);
+              if (EV) {
+                ev += (double)(okf * (r6inv * (l3 * r6inv - l4)));
+                v0 += (double)(dx * dx * fpair);
+                v1 += (double)(dy * dy * fpair);
+                v2 += (double)(dz * dz * fpair);
+                v3 += (double)(dx * dy * fpair);
+                v4 += (double)(dx * dz * fpair);
+                v5 += (double)(dy * dz * fpair);
               }
             });
       } else {
         const double4 pi = a.site[k];
         walk_row<TPR, UNROLL, double4>(
             row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
-            [&](unsigned ent, const double4& sj) {
+            [&](unsigned ent, const double4& sj, bool valid) {
               double shx, shy, shz;
               img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
               const double delx = pi.x - (sj.x + shx);
               const double dely = pi.y - (sj.y + shy);
               const double delz = pi.z - (sj.z + shz);
-              const double rsq = delx * delx + dely * dely + delz * delz;
-              if (rsq < a.cutsq) {
-                const double r2inv = 1.0 / rsq;
-                const double r6inv = r2inv * r2inv * r2inv;
-                const double forcelj = r6inv * (q.lj1 * r6inv - q.lj2);
-                const double fpair = forcelj * r2inv;
-                fx += delx * fpair;
-                fy += dely * fpair;
-                fz += delz * fpair;
-                if (EV) {
-                  ev += r6inv * (q.lj3 * r6inv - q.lj4);
-                  v0 += delx * delx * fpair;
-                  v1 += dely * dely * fpair;
-                  v2 += delz * delz * fpair;
-                  v3 += delx * dely * fpair;
-                  v4 += delx * delz * fpair;
-                  v5 += dely * delz * fpair;
-                }
+              const double rsq = fmax(delx * delx + dely * dely + delz * delz, 1e-200);
+              const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
+              const double r2inv = 1.0 / rsq;
+              const double r6inv = r2inv * r2inv * r2inv;
+              const double forcelj = r6inv * (q.lj1 * r6inv - q.lj2);
+              const double fpair = okf * (forcelj * r2inv);
+              fx += delx * fpair;
+              fy += dely * fpair;
+              fz += delz * fpair;
+              if (EV) {
+                ev += okf * (r6inv * (q.lj3 * r6inv - q.lj4));
+                v0 += delx * delx * fpair;
+                v1 += dely * dely * fpair;
+                v2 += delz * delz * fpair;
+                v3 += delx * dely * fpair;
+                v4 += delx * delz * fpair;
+                v5 += dely * delz * fpair;
               }
             });
       }
@@ -715,9 +779,9 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
 
 static PairArgs pair_args(Ctx& c, double cutsq, int rows_per_wg) {
   PairArgs a;
-  a.site = c.site.get();
+  a.site = c.site_live.get();
   a.site32 = c.site32.get();
-  a.row_site = c.row_site.get();
+  a.row_site = c.row_live.get();
   a.row_point = c.row_point.get();
   a.nnum = c.innum.get();
   a.nlist = reinterpret_cast<const unsigned*>(c.ilist.get());
@@ -781,12 +845,9 @@ static void run_eam(Ctx& c, int eflag, int vflag, Gate g) {
   }
   {
     ScopedTimer tm(c, "eam_embed");
-    const int nuc = c.nucell > 0 ? c.nucell : 1;
-    k_embed<MIXED><<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.rho.get(), ucell_grid(c), c.sdesc.get(), c.S, c.A, nuc,
-                                                             c.frho_spl.get(), c.rdrho, c.nrho,
-                                                             MIXED ? nullptr : c.site.get(),
-                                                             MIXED ? c.site32.get() : nullptr, c.live.get(),
-                                                             c.live_gen.get(), g);
+    k_embed<MIXED><<<grid_for(c.S, 256 * RSPT), 256, 0, c.stream>>>(c.rho.get(), ucell_grid(c), c.lsdesc.get(),
+                                                                    c.d_nlive.get(), c.A, c.frho_spl.get(), c.rdrho,
+                                                                    c.nrho, c.site_live.get(), c.site32.get());
   }
   EamForceArgs q;
   q.tab = tab;
