@@ -118,7 +118,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--inner-skin", type=float, default=-1.0,
-                    help="prune margin delta of the inner list (A); < 0 takes 0.25 * skin")
+                    help="prune margin delta of the inner list (A); < 0 takes 0.35 * skin")
     ap.add_argument("--profile-run", action="store_true",
                     help="mixed model only, --steps steps, then a 1 GiB copy (program of the rocprofv3 summaries)")
     args = ap.parse_args()

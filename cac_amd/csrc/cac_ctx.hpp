@@ -218,7 +218,7 @@ struct Ctx {
   DBuf<int> innum, ilist;
   int icap = 0;
   double inner_skin = 0.0;       // delta in use
-  double inner_skin_set = -1.0;  // requested delta; < 0 picks 0.25 skin
+  double inner_skin_set = -1.0;  // requested delta; < 0 picks 0.35 skin
   double inner_cutsq = 0.0;
   int inner_mode = -1;       // precision the inner list was built for (-1: stale)
   long inner_builds = 0;
@@ -263,6 +263,7 @@ struct Ctx {
   // into dense arrays in site (cell) order; the inner lists and the rows index them
   DBuf<int> live_flag, live_idx, lsdesc, lsimg, lscell, row_live, d_nlive;
   DBuf<double4> site_live;        // FP64 records (x, y, z, F') of the live sites
+  DBuf<float4> site32f;           // mixed path: FP32 cell-relative records of every site (for the prune)
   volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
   int* h_steps_dev = nullptr;       // device address of h_steps
   hipEvent_t ev_check = nullptr;    // recorded after the check of a speculative step

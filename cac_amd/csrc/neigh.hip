@@ -698,11 +698,17 @@ int check_distance(Ctx& c, bool outer, int* inner) {
 constexpr int MIX_SHIFT = 25;
 
 // Each row also holds its point's position for the next inner-list check.
+// The mixed path tests distances on the FP32 cell-relative records of every
+// site, with the cutoff widened by 1e-5 relative: that covers FP32 rounding
+// of the pair vector (below 1e-6 relative at these lengths), so the inner
+// list still holds every pair within rc + delta.  The FP64 path tests the FP64
+// records.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site, const int* __restrict__ scell,
-                                               const int* __restrict__ row_site, const int* __restrict__ row_point,
-                                               const int* __restrict__ nnum, const unsigned* __restrict__ nlist,
-                                               int R, int cap, Grid g, double cutsq_in, int* __restrict__ innum,
+__global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site, const float4* __restrict__ site32f,
+                                               const int* __restrict__ scell, const int* __restrict__ row_site,
+                                               const int* __restrict__ row_point, const int* __restrict__ nnum,
+                                               const unsigned* __restrict__ nlist, int R, int cap, Grid g,
+                                               double cutsq_in, int* __restrict__ innum,
                                                unsigned* __restrict__ ilist, const double* __restrict__ x,
                                                double* __restrict__ xhold_in, int* __restrict__ live) {
   constexpr int PU = 4;  // 16-entry groups in flight per lane group
@@ -719,7 +725,10 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
   if (lane == 3) live[row_site[r]] = 1;
   const int n = nnum[r];
   const int k = row_site[r];
-  const double4 pk = site[k];
+  const double4 pk = MIXED ? make_double4(0.0, 0.0, 0.0, 0.0) : site[k];
+  const float4 pk32 = MIXED ? site32f[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float cut32 = (float)(cutsq_in * (1.0 + 1e-5));
+  const float cs0 = (float)(g.L[0] / g.nc[0]), cs1 = (float)(g.L[1] / g.nc[1]), cs2 = (float)(g.L[2] / g.nc[2]);
   const int cp = scell[k];
   const int icx = cell_ix(cp), icy = cell_iy(cp), icz = cell_iz(cp);
   const unsigned* row = nlist + (size_t)r * cap;
@@ -728,6 +737,7 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
   for (int base = 0; base < n; base += 16 * PU) {
     unsigned ent[PU];
     double4 sj[PU];
+    float4 sj32[PU];
     int cj[PU];
 #pragma unroll
     for (int u = 0; u < PU; u++) {
@@ -736,25 +746,35 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
     }
 #pragma unroll
     for (int u = 0; u < PU; u++) {
-      sj[u] = site[ent[u] & SITE_MASK];
-      if (MIXED) cj[u] = scell[ent[u] & SITE_MASK];
+      if (MIXED) {
+        sj32[u] = site32f[ent[u] & SITE_MASK];
+        cj[u] = scell[ent[u] & SITE_MASK];
+      } else {
+        sj[u] = site[ent[u] & SITE_MASK];
+      }
     }
 #pragma unroll
     for (int u = 0; u < PU; u++) {
       const int jj = base + 16 * u + lane;
       const unsigned t = ent[u] >> IMG_SHIFT;
       const int tx = (int)(t % 3u) - 1, ty = (int)((t / 3u) % 3u) - 1, tz = (int)(t / 9u) - 1;
-      const double dx = pk.x - (sj[u].x + (double)tx * g.L[0]);
-      const double dy = pk.y - (sj[u].y + (double)ty * g.L[1]);
-      const double dz = pk.z - (sj[u].z + (double)tz * g.L[2]);
-      const bool keep = jj < n && dx * dx + dy * dy + dz * dz <= cutsq_in;
+      bool keep;
       unsigned eout = ent[u];
       if (MIXED) {
         const int ox = cell_ix(cj[u]) + tx * g.nc[0] - icx;
         const int oy = cell_iy(cj[u]) + ty * g.nc[1] - icy;
         const int oz = cell_iz(cj[u]) + tz * g.nc[2] - icz;
+        const float dx = (pk32.x - sj32[u].x) - (float)ox * cs0;
+        const float dy = (pk32.y - sj32[u].y) - (float)oy * cs1;
+        const float dz = (pk32.z - sj32[u].z) - (float)oz * cs2;
+        keep = jj < n && dx * dx + dy * dy + dz * dz <= cut32;
         const unsigned dcode = (unsigned)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
         eout = (ent[u] & SITE_MASK) | (dcode << MIX_SHIFT);
+      } else {
+        const double dx = pk.x - (sj[u].x + (double)tx * g.L[0]);
+        const double dy = pk.y - (sj[u].y + (double)ty * g.L[1]);
+        const double dz = pk.z - (sj[u].z + (double)tz * g.L[2]);
+        keep = jj < n && dx * dx + dy * dy + dz * dz <= cutsq_in;
       }
       const unsigned long long m = __ballot(keep);
       const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
@@ -812,13 +832,13 @@ void build_inner(Ctx& c) {
   const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
   unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());
   if (mixed)
-    k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.row_point.get(),
-                                                 c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
-                                                 c.x.get(), c.xhold_in.get(), c.live_flag.get());
+    k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.scell.get(), c.row_site.get(),
+                                                 c.row_point.get(), c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq,
+                                                 c.innum.get(), il, c.x.get(), c.xhold_in.get(), c.live_flag.get());
   else
-    k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.scell.get(), c.row_site.get(), c.row_point.get(),
-                                                  c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
-                                                  c.x.get(), c.xhold_in.get(), c.live_flag.get());
+    k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.scell.get(), c.row_site.get(),
+                                                  c.row_point.get(), c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq,
+                                                  c.innum.get(), il, c.x.get(), c.xhold_in.get(), c.live_flag.get());
   scan_exclusive(c, c.live_flag.get(), c.live_idx.get(), c.S, nullptr);
   k_live_count<<<1, 64, 0, c.stream>>>(c.live_flag.get(), c.live_idx.get(), c.S, c.d_nlive.get());
   compact_live(c);

@@ -142,8 +142,9 @@ struct Frame {
 // round trips overlap.
 //
 // Full mode (a neighbor build, or a step that re-prunes the inner list)
-// refreshes every site's FP64 record, which the list build and the prune
-// read; the dense live records then come from them (compact_live).
+// refreshes every site's record the prune reads (FP32 cell-relative in the
+// mixed path, FP64 otherwise); the dense live records then come from them
+// (compact_live).
 // Otherwise only the live sites are refreshed, straight into their dense
 // records: FP32 cell-relative in the mixed path, FP64 otherwise.
 constexpr int RSPT = 2;
@@ -151,7 +152,8 @@ constexpr int RSPT = 2;
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, UcellGrid ug, Frame fr, int S, int A,
                                                  int P, const int* __restrict__ sdesc, const int* __restrict__ simg,
-                                                 double4* __restrict__ site, const int* __restrict__ nlive,
+                                                 const int* __restrict__ scell, double4* __restrict__ site,
+                                                 float4* __restrict__ site32f, const int* __restrict__ nlive,
                                                  const int* __restrict__ lsdesc, const int* __restrict__ lsimg,
                                                  const int* __restrict__ lscell, float4* __restrict__ site32,
                                                  double4* __restrict__ site_live, int all, Gate g) {
@@ -169,7 +171,7 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, U
     const int kc = kk[t] < n ? kk[t] : 0;
     d[t] = dsc[kc];
     im[t] = img[kc];
-    cc[t] = MIXED && !full ? lscell[kc] : 0;
+    cc[t] = MIXED ? (full ? scell[kc] : lscell[kc]) : 0;
   }
 #pragma unroll
   for (int t = 0; t < RSPT; t++) {
@@ -194,14 +196,14 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, U
     img_shift((unsigned)im[t], fr.L, sh[0], sh[1], sh[2]);
     for (int c = 0; c < 3; c++) y[c] = y[c] + sh[c];
     const int k = kk[t];
-    if (full)
-      site[k] = make_double4(y[0], y[1], y[2], 0.0);
-    else if (MIXED)
-      site32[k] = make_float4((float)(y[0] - (fr.lo[0] + cell_ix(cc[t]) * fr.cs[0])),
-                              (float)(y[1] - (fr.lo[1] + cell_iy(cc[t]) * fr.cs[1])),
-                              (float)(y[2] - (fr.lo[2] + cell_iz(cc[t]) * fr.cs[2])), 0.0f);
-    else
-      site_live[k] = make_double4(y[0], y[1], y[2], 0.0);
+    if (MIXED) {
+      const float4 r32 = make_float4((float)(y[0] - (fr.lo[0] + cell_ix(cc[t]) * fr.cs[0])),
+                                     (float)(y[1] - (fr.lo[1] + cell_iy(cc[t]) * fr.cs[1])),
+                                     (float)(y[2] - (fr.lo[2] + cell_iz(cc[t]) * fr.cs[2])), 0.0f);
+      (full ? site32f : site32)[k] = r32;
+    } else {
+      (full ? site : site_live)[k] = make_double4(y[0], y[1], y[2], 0.0);
+    }
   }
 }
 
@@ -210,24 +212,22 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void k_live_compact(const int* __restrict__ flag, const int* __restrict__ idx, int S,
                                                       Frame fr, const int* __restrict__ sdesc,
                                                       const int* __restrict__ simg, const int* __restrict__ scell,
-                                                      const double4* __restrict__ site, int* __restrict__ lsdesc,
+                                                      const double4* __restrict__ site,
+                                                      const float4* __restrict__ site32f, int* __restrict__ lsdesc,
                                                       int* __restrict__ lsimg, int* __restrict__ lscell,
                                                       float4* __restrict__ site32, double4* __restrict__ site_live) {
-#pragma clang fp contract(off)
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S || !flag[k]) return;
   const int i = idx[k];
-  const int cc = scell[k];
   lsdesc[i] = sdesc[k];
   lsimg[i] = simg[k];
-  lscell[i] = cc;
-  const double4 y = site[k];
-  if (MIXED)
-    site32[i] = make_float4((float)(y.x - (fr.lo[0] + cell_ix(cc) * fr.cs[0])),
-                            (float)(y.y - (fr.lo[1] + cell_iy(cc) * fr.cs[1])),
-                            (float)(y.z - (fr.lo[2] + cell_iz(cc) * fr.cs[2])), 0.0f);
-  else
+  lscell[i] = scell[k];
+  if (MIXED) {
+    site32[i] = site32f[k];
+  } else {
+    const double4 y = site[k];
     site_live[i] = make_double4(y.x, y.y, y.z, 0.0);
+  }
 }
 
 static UcellGrid ucell_grid(const Ctx& c) {
@@ -256,10 +256,12 @@ static void reserve_live(Ctx& c) {
   c.lsdesc.reserve(c.S);
   c.lsimg.reserve(c.S);
   c.lscell.reserve(c.S);
-  if (c.precision == 1)
+  if (c.precision == 1) {
     c.site32.reserve(c.S);
-  else
+    c.site32f.reserve(c.S);
+  } else {
     c.site_live.reserve(c.S);
+  }
 }
 
 void site_refresh(Ctx& c, bool full, Gate g) {
@@ -272,9 +274,9 @@ void site_refresh(Ctx& c, bool full, Gate g) {
   const UcellGrid ug = ucell_grid(c);
   const int grid = grid_for(c.S, 256 * RSPT);
   auto kr = mixed ? k_refresh<true> : k_refresh<false>;
-  kr<<<grid, 256, 0, c.stream>>>(c.x.get(), ug, fr, c.S, c.A, c.P, c.sdesc.get(), c.simg.get(), c.site.get(),
-                                 c.d_nlive.get(), c.lsdesc.get(), c.lsimg.get(), c.lscell.get(), c.site32.get(),
-                                 c.site_live.get(), full ? 1 : 0, g);
+  kr<<<grid, 256, 0, c.stream>>>(c.x.get(), ug, fr, c.S, c.A, c.P, c.sdesc.get(), c.simg.get(), c.scell.get(),
+                                 c.site.get(), c.site32f.get(), c.d_nlive.get(), c.lsdesc.get(), c.lsimg.get(),
+                                 c.lscell.get(), c.site32.get(), c.site_live.get(), full ? 1 : 0, g);
 }
 
 void compact_live(Ctx& c) {
@@ -282,8 +284,9 @@ void compact_live(Ctx& c) {
   const Frame fr = frame_of(c);
   auto kc = c.precision == 1 ? k_live_compact<true> : k_live_compact<false>;
   kc<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.live_flag.get(), c.live_idx.get(), c.S, fr, c.sdesc.get(),
-                                               c.simg.get(), c.scell.get(), c.site.get(), c.lsdesc.get(),
-                                               c.lsimg.get(), c.lscell.get(), c.site32.get(), c.site_live.get());
+                                               c.simg.get(), c.scell.get(), c.site.get(), c.site32f.get(),
+                                               c.lsdesc.get(), c.lsimg.get(), c.lscell.get(), c.site32.get(),
+                                               c.site_live.get());
 }
 
 // ------------------------------------------------------------------ spline helpers
