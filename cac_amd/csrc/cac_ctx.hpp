@@ -45,6 +45,14 @@ __host__ __device__ inline int cell_ix(int p) { return p & ((1 << CELL_BITS_X) -
 __host__ __device__ inline int cell_iy(int p) { return (p >> CELL_BITS_X) & ((1 << CELL_BITS_Y) - 1); }
 __host__ __device__ inline int cell_iz(int p) { return (p >> (CELL_BITS_X + CELL_BITS_Y)) & ((1 << CELL_BITS_Z) - 1); }
 
+// Workgroup b of a launch runs on XCD b % 8.  xcd_block renumbers the grid so
+// that each XCD's workgroups take one contiguous range of logical blocks: data
+// that neighbouring blocks share then stays in one XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int grid) {
+  const int q = grid / 8, rem = grid % 8, x = b % 8;
+  return x * q + (x < rem ? x : rem) + b / 8;
+}
+
 // Trilinear Hex shape functions of unit cell u (i-major: u = (i ny + j) nz + k)
 // at natural coordinates (h_x i - 1, h_y j - 1, h_z k - 1), h = 2 / (n - 1),
 // formed with the same operations, in the same order and unfused, as the
@@ -208,10 +216,12 @@ struct Ctx {
   DBuf<double4> site, site2;
   DBuf<float4> site32;  // mixed path: (x, y, z) - cell corner, F'(rho)
   DBuf<int> scell, scell2;  // cell of each site (the frame of site32)
-  DBuf<double> tsx;
   DBuf<int> sdesc, simg, sdesc2, simg2, used, newidx;
-  DBuf<int> tdesc, timg, tcell, perm, row_point, row_site, ecount, eoffset, scan_tmp;
-  DBuf<int> cell_start, occ, nearA, nearB, sort_keys, sort_vals;
+  DBuf<int> tcell, row_point, row_site, ecount, eoffset, scan_tmp;
+  DBuf<int> cell_start, occ, nearA, nearB, sort_keys;
+  DBuf<unsigned long long> tval, sort_vals;  // candidate site descriptor << 32 | wrap code, unsorted / sorted
+  DBuf<unsigned long long> row_key;          // Morton keys of the rows' cells (and their sorted copy)
+  DBuf<int> row_idx, row_tmp;
   DBuf<unsigned char> sort_tmp;
   DBuf<int> nnum, nlist;
   int cap = 0;
@@ -291,7 +301,10 @@ void compact_live(Ctx& c);  // dense live-site records from the full ones (after
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
                   std::vector<int>& ucell, std::vector<int>& img);
 void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total);
-void sort_sites_by_cell(Ctx& c, const int* tcell, int S, int* perm);  // stable, by cell; fills cell_start
+// stable sort of the candidate sites by cell: sort_keys / sort_vals, cell_start
+void sort_sites_by_cell(Ctx& c, const int* tcell, const unsigned long long* tval, int S);
+void site_positions(Ctx& c);  // FP64 records of every site from its descriptor (list build)
+void order_rows(Ctx& c);      // rows in Morton order of their cells
 
 // pair.hip
 // positions of every live site; full = also the FP64 records (always kept in

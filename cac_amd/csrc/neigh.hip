@@ -231,6 +231,7 @@ __device__ __forceinline__ void shift_of(int code, const double* L, double* sh) 
 // order, as the reference forms Gauss-point positions, pair_eam.cpp:887-893)
 __device__ __forceinline__ void interp_pos(const double* __restrict__ x, const double* __restrict__ shape, int A,
                                            int P, int e, int u, double* y) {
+#pragma clang fp contract(off)
   const double* N = shape + (size_t)u * NPE;
   const size_t base = (size_t)A + (size_t)NPE * e;
   for (int d = 0; d < 3; d++) {
@@ -255,8 +256,12 @@ __global__ void k_point_occ(const double* __restrict__ x, int P, Grid g, int* __
   occ[cell] = 1;
 }
 
-__global__ void k_point_sites(const double* __restrict__ x, int P, Grid g, double* __restrict__ tsx,
-                              int* __restrict__ tdesc, int* __restrict__ timg, int* __restrict__ tcell, int S_cap) {
+__device__ __forceinline__ unsigned long long pack_site(int desc, int code) {
+  return ((unsigned long long)(unsigned)desc << 32) | (unsigned)code;
+}
+
+__global__ void k_point_sites(const double* __restrict__ x, int P, Grid g, unsigned long long* __restrict__ tval,
+                              int* __restrict__ tcell) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   double y[3] = {x[p], x[(size_t)P + p], x[2 * (size_t)P + p]};
@@ -266,9 +271,7 @@ __global__ void k_point_sites(const double* __restrict__ x, int P, Grid g, doubl
     cell = 0;
     code = 13;
   }
-  for (int d = 0; d < 3; d++) tsx[(size_t)d * S_cap + p] = y[d];
-  tdesc[p] = p;
-  timg[p] = code;
+  tval[p] = pack_site(p, code);
   tcell[p] = cell;
 }
 
@@ -301,9 +304,8 @@ template <bool FILL>
 __global__ __launch_bounds__(VB) void k_virt(const double* __restrict__ x, const double* __restrict__ shape,
                                              const int* __restrict__ u2g, const int* __restrict__ nearmask, int A,
                                              int P, int nucell, Grid g, int* __restrict__ ecount,
-                                             const int* __restrict__ eoffset, double* __restrict__ tsx,
-                                             int* __restrict__ tdesc, int* __restrict__ timg,
-                                             int* __restrict__ tcell, int S_cap) {
+                                             const int* __restrict__ eoffset,
+                                             unsigned long long* __restrict__ tval, int* __restrict__ tcell) {
   const int e = blockIdx.x;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -330,9 +332,7 @@ __global__ __launch_bounds__(VB) void k_virt(const double* __restrict__ x, const
     }
     if (FILL && need) {
       const int s = base + total + woff + rank;
-      for (int d = 0; d < 3; d++) tsx[(size_t)d * S_cap + s] = y[d];
-      tdesc[s] = -(1 + e * nucell + u);
-      timg[s] = code;
+      tval[s] = pack_site(-(1 + e * nucell + u), code);
       tcell[s] = cell;
     }
     total += chunk;
@@ -341,19 +341,18 @@ __global__ __launch_bounds__(VB) void k_virt(const double* __restrict__ x, const
   if (!FILL && t == 0) ecount[e] = total;
 }
 
-__global__ void k_apply_perm(const int* __restrict__ perm, int S, int S_cap, Grid g, const double* __restrict__ tsx,
-                             const int* __restrict__ tdesc, const int* __restrict__ timg,
-                             const int* __restrict__ tcell, double4* __restrict__ site, int* __restrict__ sdesc,
-                             int* __restrict__ simg, int* __restrict__ scell, int* __restrict__ isrow) {
+// sorted (cell, descriptor) pairs -> site descriptors, packed cell coordinates, row flags
+__global__ void k_unpack_sites(const int* __restrict__ key, const unsigned long long* __restrict__ val, int S, Grid g,
+                               int* __restrict__ sdesc, int* __restrict__ simg, int* __restrict__ scell,
+                               int* __restrict__ isrow) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S) return;
-  const int s = perm[k];
-  const int cid = tcell[s];
+  const int cid = key[k];
+  const unsigned long long v = val[k];
+  const int desc = (int)(unsigned)(v >> 32);
   scell[k] = cell_pack(cid % g.nc[0], (cid / g.nc[0]) % g.nc[1], cid / (g.nc[0] * g.nc[1]));
-  site[k] = make_double4(tsx[s], tsx[(size_t)S_cap + s], tsx[2 * (size_t)S_cap + s], 0.0);
-  const int desc = tdesc[s];
   sdesc[k] = desc;
-  simg[k] = timg[s];
+  simg[k] = (int)(unsigned)(v & 0xffffffffull);
   isrow[k] = desc >= 0;
 }
 
@@ -383,7 +382,7 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
 #pragma clang fp contract(off)
   const int lane = threadIdx.x % BL;
   const int gbase = (threadIdx.x & 63) & ~(BL - 1);
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / BL);
+  const int r = (int)(((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / BL);
   if (r >= R) return;
   const int k = row_site[r];
   const double4 pk = site[k];
@@ -458,7 +457,7 @@ __global__ __launch_bounds__(256) void k_mark_used(const int* __restrict__ row_s
                                                    const unsigned* __restrict__ nlist, int R, int cap,
                                                    int* __restrict__ used) {
   const int lane = threadIdx.x & 15;
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
+  const int r = (int)(((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 4);
   if (r >= R) return;
   if (lane == 0) used[row_site[r]] = 1;
   const unsigned* row = nlist + (size_t)r * cap;
@@ -484,7 +483,7 @@ __global__ __launch_bounds__(256) void k_remap_rows(int* __restrict__ row_site, 
                                                     unsigned* __restrict__ nlist, int R, int cap,
                                                     const int* __restrict__ newidx) {
   const int lane = threadIdx.x & 15;
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4);
+  const int r = (int)(((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 4);
   if (r >= R) return;
   if (lane == 0) row_site[r] = newidx[row_site[r]];
   unsigned* row = nlist + (size_t)r * cap;
@@ -545,45 +544,40 @@ void build_neighbor(Ctx& c) {
     c.ecount.reserve(c.E);
     c.eoffset.reserve(c.E);
     k_virt<false><<<c.E, VB, 0, c.stream>>>(c.x.get(), c.shape.get(), c.u2g.get(), c.nearA.get(), c.A, P, c.nucell,
-                                           g, c.ecount.get(), nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+                                           g, c.ecount.get(), nullptr, nullptr, nullptr);
     scan_exclusive(c, c.ecount.get(), c.eoffset.get(), c.E, &nvirt);
   }
   const int S = P + nvirt;
   if ((long)S >= (1L << IMG_SHIFT)) throw HipError{hipErrorInvalidValue, "more than 2^27 sites"};
-  const int S_cap = S;
-  c.tsx.reserve((size_t)3 * S_cap);
-  c.tdesc.reserve(S_cap);
-  c.timg.reserve(S_cap);
-  c.tcell.reserve(S_cap);
-  k_point_sites<<<grid_for(P, 256), 256, 0, c.stream>>>(c.x.get(), P, g, c.tsx.get(), c.tdesc.get(), c.timg.get(),
-                                                        c.tcell.get(), S_cap);
+  c.tval.reserve(S);
+  c.tcell.reserve(S);
+  k_point_sites<<<grid_for(P, 256), 256, 0, c.stream>>>(c.x.get(), P, g, c.tval.get(), c.tcell.get());
   if (c.E && nvirt)
     k_virt<true><<<c.E, VB, 0, c.stream>>>(c.x.get(), c.shape.get(), c.u2g.get(), c.nearA.get(), c.A, P, c.nucell, g,
-                                          nullptr, c.eoffset.get(), c.tsx.get(), c.tdesc.get(), c.timg.get(),
-                                          c.tcell.get(), S_cap);
+                                          nullptr, c.eoffset.get(), c.tval.get(), c.tcell.get());
 
-  // sites ordered by cell (stable: candidate order inside a cell)
-  c.perm.reserve(S);
-  sort_sites_by_cell(c, c.tcell.get(), S, c.perm.get());
-  c.site.reserve(S_cap);
-  c.sdesc.reserve(S_cap);
-  c.simg.reserve(S_cap);
-  c.isflag.reserve(S_cap);
+  // sites ordered by cell (stable: candidate order inside a cell), then their
+  // FP64 records from the descriptors, with the refresh's arithmetic
+  sort_sites_by_cell(c, c.tcell.get(), c.tval.get(), S);
+  c.site.reserve(S);
+  c.sdesc.reserve(S);
+  c.simg.reserve(S);
+  c.isflag.reserve(S);
   c.row_point.reserve(P);
   c.row_site.reserve(P);
-  c.scell.reserve(S_cap);
-  k_apply_perm<<<grid_for(S, 256), 256, 0, c.stream>>>(c.perm.get(), S, S_cap, g, c.tsx.get(), c.tdesc.get(),
-                                                       c.timg.get(), c.tcell.get(), c.site.get(), c.sdesc.get(),
-                                                       c.simg.get(), c.scell.get(), c.isflag.get());
+  c.scell.reserve(S);
+  k_unpack_sites<<<grid_for(S, 256), 256, 0, c.stream>>>(c.sort_keys.get(), c.sort_vals.get(), S, g, c.sdesc.get(),
+                                                         c.simg.get(), c.scell.get(), c.isflag.get());
+  c.S = S;
+  site_positions(c);
   c.rowidx.reserve(S);
   int nrows = 0;
   scan_exclusive(c, c.isflag.get(), c.rowidx.get(), S, &nrows);
   if (nrows != P) throw HipError{hipErrorInvalidValue, "row count mismatch"};
   k_rows<<<grid_for(S, 256), 256, 0, c.stream>>>(c.sdesc.get(), c.isflag.get(), c.rowidx.get(), S,
                                                  c.row_point.get(), c.row_site.get());
-  c.S = S;
+  order_rows(c);
   c.nvirt = nvirt;
-  (void)S_cap;
 
   // full lists; grow the per-point capacity and rebuild once if a row overflowed
   if (c.cap == 0) {  // first build: the mean count at the site density, with margin
@@ -716,7 +710,8 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
   const int gbase = (threadIdx.x & 63) & ~15;
   // grid-stride over rows, so a step that does not re-prune launches few workgroups
   const long rstride = ((long)gridDim.x * blockDim.x) >> 4;
-  for (long rl = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4; rl < R; rl += rstride) {
+  const long rfirst = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 4;
+  for (long rl = rfirst; rl < R; rl += rstride) {
   const int r = (int)rl;
   if (lane < 3) {
     const size_t i = (size_t)lane * R + row_point[r];
@@ -801,7 +796,8 @@ __global__ __launch_bounds__(256) void k_live_remap(const int* __restrict__ idx,
                                                     unsigned* __restrict__ ilist, int R, int cap, unsigned mask) {
   const int lane = threadIdx.x & 15;
   const long rstride = ((long)gridDim.x * blockDim.x) >> 4;
-  for (long rl = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4; rl < R; rl += rstride) {
+  const long rfirst = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 4;
+  for (long rl = rfirst; rl < R; rl += rstride) {
     const int r = (int)rl;
     if (lane == 0) row_live[r] = idx[row_site[r]];
     unsigned* row = ilist + (size_t)r * cap;

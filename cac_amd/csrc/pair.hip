@@ -279,6 +279,14 @@ void site_refresh(Ctx& c, bool full, Gate g) {
                                  c.lscell.get(), c.site32.get(), c.site_live.get(), full ? 1 : 0, g);
 }
 
+void site_positions(Ctx& c) {
+  const Frame fr = frame_of(c);
+  const UcellGrid ug = ucell_grid(c);
+  k_refresh<false><<<grid_for(c.S, 256 * RSPT), 256, 0, c.stream>>>(
+      c.x.get(), ug, fr, c.S, c.A, c.P, c.sdesc.get(), c.simg.get(), c.scell.get(), c.site.get(), nullptr, nullptr,
+      nullptr, nullptr, nullptr, nullptr, nullptr, 1, Gate());
+}
+
 void compact_live(Ctx& c) {
   reserve_live(c);
   const Frame fr = frame_of(c);
