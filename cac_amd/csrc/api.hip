@@ -219,6 +219,7 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     CAC_HIP(hipEventCreateWithFlags(&c.ev_check, hipEventDisableTiming));
     c.units = units;
     if (const char* e = std::getenv("CAC_HIP_TPR")) c.tpr = std::atoi(e);
+    if (const char* e = std::getenv("CAC_HIP_TPR_FORCE")) c.tpr_force = std::atoi(e);
     if (const char* e = std::getenv("CAC_HIP_GRID")) c.grid_cap = std::max(8, std::atoi(e));
     if (const char* e = std::getenv("CAC_HIP_PAIR_SHAPE")) c.pair_shape = std::atoi(e);
     CAC_HIP(hipDeviceGetAttribute(&c.ncu, hipDeviceAttributeMultiprocessorCount, device));

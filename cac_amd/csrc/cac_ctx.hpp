@@ -255,7 +255,8 @@ struct Ctx {
   DBuf<float> tab32f;  // mixed force pass, per knot: z2r c3..c6, then rhor c0..c2 and z2r c2 (FP32)
   DBuf<double> tab64;  // the same in FP64
   int precision = 0;     // 0 FP64, 1 mixed
-  int tpr = 8;           // lanes per neighbor row in the pair kernels
+  int tpr = 8;           // lanes per neighbor row in the density pass
+  int tpr_force = 4;     // lanes per neighbor row in the force passes (EAM, LJ)
   int grid_cap = 4096;   // max workgroups of the persistent pair kernels
   int pair_shape = 1;    // EAM pass shape: 0/1/2/3 = (512|1024 threads) x (4|8 entries in flight)
   int ncu = 256;         // compute units of the device

@@ -837,31 +837,28 @@ static int resident_grid(Ctx& c, K kernel, int nt, size_t lds, long nchunks) {
 }
 
 template <int TPR, bool MIXED, int NT, int U>
-static void run_eam(Ctx& c, int eflag, int vflag, Gate g) {
+static void run_density(Ctx& c, Gate g) {
+  const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
+  const size_t lds_rho = knot * (c.nr + 1) + (MIXED ? sizeof(float4) * NOFF : 0);
+  const PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
+  const void* tab_rho = MIXED ? (const void*)c.tab32r.get() : (const void*)c.tab64.get();
+  ScopedTimer tm(c, "eam_density");
+  auto kd = k_eam_density<TPR, MIXED, NT, U>;
+  const int grid = resident_grid(c, kd, NT, lds_rho, a.nchunks);
+  kd<<<grid, NT, lds_rho, c.stream>>>(a, tab_rho, c.rdr, c.nr, c.rho.get(), g);
+}
+
+template <int TPR, bool MIXED, int NT, int U>
+static void run_force(Ctx& c, int eflag, int vflag, Gate g) {
   const bool ev = eflag || vflag;
   const int E8 = NPE * c.E;
   double* gf = c.gf.get();
   double* er = ev ? c.erow.get() : nullptr;
   const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
-  const size_t offs = MIXED ? sizeof(float4) * NOFF : 0;
-  const size_t lds_rho = knot * (c.nr + 1) + offs, lds_force = 2 * knot * (c.nr + 1) + offs;
-  PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
-  const void* tab_rho = MIXED ? (const void*)c.tab32r.get() : (const void*)c.tab64.get();
-  const void* tab = MIXED ? (const void*)c.tab32f.get() : (const void*)c.tab64.get();
-  {
-    ScopedTimer tm(c, "eam_density");
-    auto kd = k_eam_density<TPR, MIXED, NT, U>;
-    const int grid = resident_grid(c, kd, NT, lds_rho, a.nchunks);
-    kd<<<grid, NT, lds_rho, c.stream>>>(a, tab_rho, c.rdr, c.nr, c.rho.get(), g);
-  }
-  {
-    ScopedTimer tm(c, "eam_embed");
-    k_embed<MIXED><<<grid_for(c.S, 256 * RSPT), 256, 0, c.stream>>>(c.rho.get(), ucell_grid(c), c.lsdesc.get(),
-                                                                    c.d_nlive.get(), c.A, c.frho_spl.get(), c.rdrho,
-                                                                    c.nrho, c.site_live.get(), c.site32.get());
-  }
+  const size_t lds_force = 2 * knot * (c.nr + 1) + (MIXED ? sizeof(float4) * NOFF : 0);
+  const PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
   EamForceArgs q;
-  q.tab = tab;
+  q.tab = MIXED ? (const void*)c.tab32f.get() : (const void*)c.tab64.get();
   q.frho = c.frho_spl.get();
   q.rho = c.rho.get();
   q.rdr = c.rdr;
@@ -883,37 +880,67 @@ static void run_eam(Ctx& c, int eflag, int vflag, Gate g) {
   }
 }
 
+// the three EAM passes; lanes per row chosen separately for the density and
+// the force pass
+template <bool MIXED, int NT, int U>
+static void run_eam(Ctx& c, int eflag, int vflag, Gate g) {
+  switch (c.tpr) {
+    case 4: run_density<4, MIXED, NT, U>(c, g); break;
+    case 16: run_density<16, MIXED, NT, U>(c, g); break;
+    default: run_density<8, MIXED, NT, U>(c, g); break;
+  }
+  {
+    ScopedTimer tm(c, "eam_embed");
+    k_embed<MIXED><<<grid_for(c.S, 256 * RSPT), 256, 0, c.stream>>>(c.rho.get(), ucell_grid(c), c.lsdesc.get(),
+                                                                    c.d_nlive.get(), c.A, c.frho_spl.get(), c.rdrho,
+                                                                    c.nrho, c.site_live.get(), c.site32.get());
+  }
+  switch (c.tpr_force) {
+    case 8: run_force<8, MIXED, NT, U>(c, eflag, vflag, g); break;
+    case 16: run_force<16, MIXED, NT, U>(c, eflag, vflag, g); break;
+    default: run_force<4, MIXED, NT, U>(c, eflag, vflag, g); break;
+  }
+}
+
 template <int TPR>
-static void launch_pair(Ctx& c, int eflag, int vflag, Gate g) {
+static void run_lj(Ctx& c, int eflag, int vflag, Gate g) {
   const bool ev = eflag || vflag;
   const bool mixed = c.precision == 1;
   const int E8 = NPE * c.E;
-  double* gf = c.gf.get();
   double* er = ev ? c.erow.get() : nullptr;
+  PairArgs a = pair_args(c, c.ljcutsq, PB / TPR);
+  LjArgs q{c.lj1, c.lj2, c.lj3, c.lj4, c.nodal_weight / 2.0, c.nodal_weight / 2.0};
+  ScopedTimer tm(c, "lj_force");
+  auto kl = mixed ? (ev ? k_lj_force<TPR, true, true> : k_lj_force<TPR, true, false>)
+                  : (ev ? k_lj_force<TPR, false, true> : k_lj_force<TPR, false, false>);
+  const int grid = resident_grid(c, kl, PB, 0, a.nchunks);
+  kl<<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), c.gf.get(), c.P, E8, er, g);
+}
+
+static void launch_pair(Ctx& c, int eflag, int vflag, Gate g) {
+  const bool mixed = c.precision == 1;
   if (c.pairstyle == 2) {
     // workgroup size and entries in flight per lane of the EAM passes
     if (mixed) {
       switch (c.pair_shape) {
-        case 1: run_eam<TPR, true, 1024, 4>(c, eflag, vflag, g); break;
-        case 2: run_eam<TPR, true, 512, 8>(c, eflag, vflag, g); break;
-        case 3: run_eam<TPR, true, 1024, 8>(c, eflag, vflag, g); break;
-        default: run_eam<TPR, true, 512, 4>(c, eflag, vflag, g); break;
+        case 1: run_eam<true, 1024, 4>(c, eflag, vflag, g); break;
+        case 2: run_eam<true, 512, 8>(c, eflag, vflag, g); break;
+        case 3: run_eam<true, 1024, 8>(c, eflag, vflag, g); break;
+        default: run_eam<true, 512, 4>(c, eflag, vflag, g); break;
       }
     } else {
       switch (c.pair_shape) {
         case 2:
-        case 3: run_eam<TPR, false, 1024, 8>(c, eflag, vflag, g); break;
-        default: run_eam<TPR, false, 1024, 4>(c, eflag, vflag, g); break;
+        case 3: run_eam<false, 1024, 8>(c, eflag, vflag, g); break;
+        default: run_eam<false, 1024, 4>(c, eflag, vflag, g); break;
       }
     }
   } else {
-    PairArgs a = pair_args(c, c.ljcutsq, PB / TPR);
-    LjArgs q{c.lj1, c.lj2, c.lj3, c.lj4, c.nodal_weight / 2.0, c.nodal_weight / 2.0};
-    ScopedTimer tm(c, "lj_force");
-    auto kl = mixed ? (ev ? k_lj_force<TPR, true, true> : k_lj_force<TPR, true, false>)
-                    : (ev ? k_lj_force<TPR, false, true> : k_lj_force<TPR, false, false>);
-    const int grid = resident_grid(c, kl, PB, 0, a.nchunks);
-    kl<<<grid, PB, 0, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er, g);
+    switch (c.tpr_force) {
+      case 8: run_lj<8>(c, eflag, vflag, g); break;
+      case 16: run_lj<16>(c, eflag, vflag, g); break;
+      default: run_lj<4>(c, eflag, vflag, g); break;
+    }
   }
 }
 
@@ -924,12 +951,7 @@ void pair_passes(Ctx& c, int eflag, int vflag, Gate g) {
     site_refresh(c, true);
     build_inner(c);
   }
-  switch (c.tpr) {
-    case 4: launch_pair<4>(c, eflag, vflag, g); break;
-    case 8: launch_pair<8>(c, eflag, vflag, g); break;
-    case 32: launch_pair<32>(c, eflag, vflag, g); break;
-    default: launch_pair<16>(c, eflag, vflag, g); break;
-  }
+  launch_pair(c, eflag, vflag, g);
 }
 
 void pair_sums(Ctx& c, int eflag, int vflag) {
