@@ -70,6 +70,21 @@ def cpu_baseline(deck, tab, nsteps):
     return t_setup, t_run
 
 
+def nve_drift(cac_amd, deck, tab, precision, steps, every=100):
+    """Total-energy trace of an NVE run: drift = slope of a linear fit of
+    E_total(t) per atom-equivalent (eV / atom / ps) and the largest excursion
+    |E(t) - E(0)| / |E(0)|."""
+    import numpy as np
+    m = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=precision)
+    m.setup()
+    th = m.run(steps, DT, thermo_every=every)
+    m.close()
+    t, e = th[:, 0] * DT, th[:, 3]
+    slope = float(np.polyfit(t, e, 1)[0]) / deck.atom_equivalents
+    return {"drift_eV_per_atom_ps": slope, "max_rel_excursion": float(np.max(np.abs(e - e[0])) / abs(e[0])),
+            "e0": float(e[0]), "e_end": float(e[-1])}
+
+
 def force_bytes(P, inner_mean, live_sites):
     """Algorithmic HBM bytes of one mixed EAM force launch.
 
@@ -119,6 +134,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--inner-skin", type=float, default=-1.0,
                     help="prune margin delta of the inner list (A); < 0 takes 0.35 * skin")
+    ap.add_argument("--drift-steps", type=int, default=10000,
+                    help="NVE steps of the FP64-vs-mixed drift comparison on the 32,000-atom EAM block (0: skip)")
     ap.add_argument("--profile-run", action="store_true",
                     help="mixed model only, --steps steps, then a 1 GiB copy (program of the rocprofv3 summaries)")
     args = ap.parse_args()
@@ -199,6 +216,16 @@ def main():
     achieved = bytes_force / (force_us * 1e-6) / 1e9
     tr = traffic_from_summary() if args.workload == "cac_eam_x8" else None
 
+    drift = None
+    if args.drift_steps > 0:
+        blk = decks.eam_block()
+        d64 = nve_drift(cac_amd, blk, tab, cac_amd.FP64, args.drift_steps)
+        dmx = nve_drift(cac_amd, blk, tab, cac_amd.MIXED, args.drift_steps)
+        drift = {"deck": "EAM Cu FCC 20^3 periodic block, 32,000 atoms, 300 K, dt 1 fs", "steps": args.drift_steps,
+                 "fp64": d64, "mixed": dmx,
+                 "ratio_mixed_over_fp64": abs(dmx["drift_eV_per_atom_ps"]) / max(abs(d64["drift_eV_per_atom_ps"]),
+                                                                                   1e-300)}
+
     cpu = None
     if not args.no_cpu_baseline:
         t_setup, t_run = cpu_baseline(deck, tab, args.cpu_steps)
@@ -235,6 +262,7 @@ def main():
                      "traffic": tr["bytes_per_launch"] if tr else None,
                      "algorithmic_bytes": bytes_force,
                      "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if tr else None},
+        "nve_drift": drift,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -333,6 +333,21 @@ def lj_block(n: int = 20, a: float = 1.5874, jitter: float = 0.01, T: float = 0.
                 atype=np.ones(x.shape[0], dtype=np.int32), name=f"lj_fcc_{n}")
 
 
+def eam_block(n: int = 20, a: float = 3.615, jitter: float = 0.02, T: float = 300.0, seed: int = 2024,
+              mass: float = 63.546) -> Deck:
+    """Pure-atomistic EAM Cu: periodic FCC n^3 cells (n = 20: 32,000 atoms, the
+    atomistic core of the 1x CAC workload as a periodic block), Gaussian jitter,
+    Maxwell velocities at T with zero net momentum."""
+    rng = np.random.default_rng(seed)
+    x = fcc_sites(a, (n, n, n))
+    x = x + rng.normal(0.0, jitter, size=x.shape)
+    L = n * a
+    x = x - L * np.floor(x / L)
+    v, _ = maxwell(rng, x.shape[0], 0, mass, T, BOLTZ_METAL, MVV2E_METAL, 1)
+    return Deck(units="metal", boxlo=np.zeros(3), boxhi=np.full(3, L), masses=np.array([mass]), x=x, v=v,
+                atype=np.ones(x.shape[0], dtype=np.int32), name=f"eam_fcc_{n}")
+
+
 def cac_eam(box_cells=(7, 8, 8), core_cells: int = 44, ncell: int = 9, jitter: float = 0.02, T: float = 300.0,
             seed: int = 2024, a: float = 3.615, mass: float = 63.546, jitter_atoms: bool = True) -> Deck:
     """EAM CAC deck on the void-free geometry of :func:`cac_geometry`.  Defaults
