@@ -191,6 +191,25 @@ def test_mixed_precision_force_error():
     assert abs(e32 - e64) < 1e-6 * abs(e64)
 
 
+def test_mixed_nve_drift_within_twice_fp64():
+    """NVE on a 4,000-atom EAM block at 300 K, 2,000 steps of 1 fs: the mixed
+    path's total-energy excursion and drift stay within twice FP64's."""
+    import cac_amd
+    deck = decks.eam_block(n=10, seed=31)
+    res = {}
+    for prec in (cac_amd.FP64, cac_amd.MIXED):
+        g = gpu_model(deck, "eam", 2.0, precision=prec)
+        g.setup()
+        th = g.run(2000, 0.001, thermo_every=50)
+        g.close()
+        t, e = th[:, 0] * 0.001, th[:, 3]
+        res[prec] = (float(np.max(np.abs(e - e[0]))), abs(float(np.polyfit(t, e, 1)[0])))
+    (x64, d64), (x32, d32) = res[cac_amd.FP64], res[cac_amd.MIXED]
+    floor = 1e-9 * deck.natoms
+    assert x32 <= 2 * x64 + floor, res
+    assert d32 <= 2 * d64 + floor, res
+
+
 # ------------------------------------------------------------------ host handover, closed forms, launch shapes
 
 def test_host_handover_matches_oracle():
