@@ -7,19 +7,22 @@
 //           (Hex node order).  x/v/f are [3][P] planes; f holds atom forces and
 //           assembled nodal forces (nodef).  gf [3][8E] is the Gauss-cell force at
 //           each node (2x2x2 Gauss cells sit on the nodes).
-//   sites   S = P + the virtual atoms some list references, in cell order.  site[S]
-//           is a 32-byte record (x, y, z, F'(rho)): the position wrapped into the
-//           box and, for EAM, the embedding derivative at the site, so one pair
-//           needs one 32-byte gather.  sdesc names the source (point index >= 0, or
-//           -(1 + e*nucell + u) for unit cell u of element e); simg is the box
-//           shift applied when wrapping.
-//   rows    R = P points in cell order; nlist is row-major [R][cap] of unsigned
-//           (site | image << 27), image = (tx+1) + 3(ty+1) + 9(tz+1): every site
-//           image within rc + skin (the reference's list).  ilist [R][icap] is the
-//           inner list the force passes walk: the entries within rc + delta,
-//           pruned on device and rebuilt whenever a point has moved delta/2
-//           since (FP64: same entry format; mixed: site | cell offset << 25 into
-//           site32, FP32 positions relative to each site's cell corner).
+//   sites   S = P + the virtual atoms some outer list references, in cell order.
+//           sdesc names the source (point index >= 0, or -(1 + e*nucell + u) for
+//           unit cell u of element e); simg is the box shift applied when
+//           wrapping; scell the site's neighbor cell.  Full position records
+//           (site, FP64; site32f, FP32 cell-relative) are written on prune steps.
+//   live    the sites some inner list reads, compacted after each prune into
+//           dense arrays in cell order (lsdesc/lsimg/lscell).  site_live (FP64)
+//           or site32 (mixed, relative to the site's cell corner) is a record
+//           (x, y, z, F'(rho)), so one pair needs one gather.
+//   rows    R = P points in Morton order of their cell; nlist is row-major
+//           [R][cap] of unsigned (site | image << 27), image = (tx+1) + 3(ty+1)
+//           + 9(tz+1): every site image within rc + skin (the reference's list).
+//           ilist [R][icap] is the inner list the force passes walk: the entries
+//           within rc + delta, pruned on device and rebuilt whenever a point has
+//           moved delta/2 since, indexing the live records (FP64: live site |
+//           image << 27; mixed: live site | cell offset << 25).
 #pragma once
 
 #include <hip/hip_runtime.h>
