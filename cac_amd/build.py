@@ -58,7 +58,8 @@ def build_host(verbose: bool = False) -> None:
     """The C++ host layer (Pair / Neighbor / FixNVE / Verlet mirrors, data-file
     reader) as libcachost.so over libcachip.so, and the cac_run driver."""
     steps = [
-        (["g++", *HOSTFLAGS, "-shared", "-o", HOSTLIB + ".tmp", os.path.join(HOST, "cac_host.cpp"), f"-L{HERE}",
+        (["g++", *HOSTFLAGS, "-shared", "-o", HOSTLIB + ".tmp", os.path.join(HOST, "cac_host.cpp"),
+          os.path.join(HOST, "cac_dump.cpp"), f"-L{HERE}",
           "-lcachip", "-Wl,-rpath,$ORIGIN"], HOSTLIB),
         (["g++", *HOSTFLAGS, "-o", RUNNER + ".tmp", os.path.join(HOST, "cac_run.cpp"), f"-L{HERE}", "-lcachost",
           "-lcachip", "-Wl,-rpath,$ORIGIN"], RUNNER),

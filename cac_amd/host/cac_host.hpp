@@ -175,6 +175,53 @@ class VerletHIP {
   FixNVEHIP& fix_;
 };
 
+// ------------------------------------------------------------------ output taps (cac_dump.cpp)
+
+// "dump ID all custom N file [format F] [average yes|no] [wrap yes|no]
+// [precision high|normal] col ..." in the LAMMPS text format
+// (dump_custom.cpp:30-121, 1260-1420, 1672-1719).  Formats lammps/full/map
+// (default), lammps/surface, lammps/gauss/point, lammps/node, lammps/center;
+// columns id type x y z vx vy vz fx fy fz mass proc procp1 eid etype ibasis
+// igauss inode iucell.  Real atoms come first, then each element's virtual
+// atoms, whose velocity and force columns are interpolated from nodev / nodef.
+// A file name with '*' gives one file per snapshot; otherwise snapshots are
+// appended to one file.
+class DumpCustom {
+ public:
+  DumpCustom(int narg, char** arg);
+  void write(const System& s, long ntimestep);
+  long count(const System& s) const;  // lines per snapshot
+  std::string id, file, columns;
+  int nevery = 1;
+
+ private:
+  enum Style { FULLMAP, SURFACE, GAUSSPOINT, NODE, CENTER };
+  enum Col { ID, TYPE, X, Y, Z, VX, VY, VZ, FX, FY, FZ, MASS, PROC, PROCP1, EID, ETYPE, IBASIS, IGAUSS, INODE, IUCELL };
+  void row(std::vector<double>& buf, const System& s, int kind, int i, int ibasis, int iucell, int igcell, int inode,
+           long vid, const double* x) const;
+  Style style = FULLMAP;
+  std::vector<Col> cols;
+  std::vector<bool> col_int;
+  bool wrap = false, precision_high = false, opened_ = false;
+};
+
+// The device's full lists by point (atoms, then 8 Gauss cells per element):
+// entries first[p] .. first[p+1]-1 are (kind 0 atom / 1 virtual atom, owner,
+// unit cell, image shift); the neighbor sits at owner's position + img * L.
+struct NeighborRows {
+  std::vector<long> first;
+  std::vector<int> kind, owner, ucell, img;
+};
+NeighborRows export_neighbor_rows(Device& dev);
+
+// "write_dump_neighbor_debug file [gauss id ibasis igcell | atom id]"
+// (write_dump_neighbor_debug.cpp:41-409): every row with its list length, or
+// one row's neighbors with their positions, tags and unit cells.  The device
+// keeps full rows, so numneigh and a row's entries are those of the full
+// list: the reference's half row of a point is a subset.  Returns the path
+// written ("" when the point does not exist, as the reference writes nothing).
+std::string write_dump_neighbor_debug(const System& s, const NeighborRows& nl, int narg, char** arg, long ntimestep);
+
 }  // namespace cac_host
 
 #endif

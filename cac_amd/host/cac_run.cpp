@@ -6,11 +6,20 @@
 //   cac_run DATA (--eam SETFL | --lj EPS SIGMA CUT) [--units metal|lj]
 //           [--skin S] [--every N] [--delay N] [--check yes|no] [--dt DT]
 //           [--steps N] [--thermo N] [--mixed] [--loop host|device]
-//   cac_run DATA --parse-only      counts and checksums of the data file
+//           [--dump "ID all custom N file ..."]... [--neighbor-debug "file [atom ID | gauss ID IB IG]"]
+//   cac_run DATA --parse-only [--dump ...]   counts and checksums of the data file
+//
+// --dump takes the arguments of the reference's dump command and writes a
+// snapshot at step 0 and every N steps; --neighbor-debug those of
+// write_dump_neighbor_debug, written after the first neighbor build.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <sstream>
 #include <string>
+#include <vector>
 
 #include "cac_host.hpp"
 
@@ -20,13 +29,29 @@ static int usage() {
   std::fprintf(stderr,
                "usage: cac_run DATA (--eam SETFL | --lj EPS SIGMA CUT) [--units metal|lj] [--skin S]\n"
                "               [--every N] [--delay N] [--check yes|no] [--dt DT] [--steps N] [--thermo N]\n"
-               "               [--mixed] [--loop host|device]\n"
-               "       cac_run DATA --parse-only\n");
+               "               [--mixed] [--loop host|device] [--dump \"ID all custom N file ...\"]...\n"
+               "               [--neighbor-debug \"file [atom ID | gauss ID IBASIS IGCELL]\"]\n"
+               "       cac_run DATA --parse-only [--dump ...]\n");
   return 2;
 }
 
+// split a command's arguments on whitespace, as the input parser hands them on
+struct Args {
+  std::vector<std::string> w;
+  std::vector<char*> p;
+  explicit Args(const std::string& line) {
+    std::istringstream in(line);
+    for (std::string t; in >> t;) w.push_back(t);
+    for (auto& t : w) p.push_back(&t[0]);
+  }
+  int n() { return (int)p.size(); }
+  char** v() { return p.data(); }
+};
+
 int main(int argc, char** argv) {
   if (argc < 3) return usage();
+  std::vector<std::string> dump_args;
+  std::string ndebug_args;
   std::string data = argv[1], eam, units = "metal", loop = "device";
   double eps = 1, sigma = 1, cut = 2.5, skin = -1, dt = -1;
   int every = 1, delay = 10, check = 1, steps = 100, thermo = 10;
@@ -49,13 +74,21 @@ int main(int argc, char** argv) {
     else if (a == "--mixed") mixed = true;
     else if (a == "--loop") { need(1); loop = argv[++i]; }
     else if (a == "--parse-only") parse_only = true;
+    else if (a == "--dump") { need(1); dump_args.push_back(argv[++i]); }
+    else if (a == "--neighbor-debug") { need(1); ndebug_args = argv[++i]; }
     else return usage();
   }
   try {
     System sys;
     sys.units = units == "lj" ? CAC_HIP_UNITS_LJ : CAC_HIP_UNITS_METAL;
     read_data(data, sys);
+    std::vector<std::unique_ptr<DumpCustom>> dumps;
+    for (auto& d : dump_args) {
+      Args a(d);
+      dumps.emplace_back(new DumpCustom(a.n(), a.v()));
+    }
     if (parse_only) {
+      for (auto& d : dumps) d->write(sys, 0);
       double sx = 0, sv = 0, sn = 0, snv = 0;
       for (double q : sys.atom.x) sx += q;
       for (double q : sys.atom.v) sv += q;
@@ -102,7 +135,40 @@ int main(int argc, char** argv) {
     FixNVEHIP fix(dev, dt);
     VerletHIP verlet(dev, *pair, neigh, fix);
     verlet.setup();
-    const auto rows = loop == "host" ? verlet.run(steps, thermo) : verlet.run_resident(steps, dt, thermo);
+    if (!ndebug_args.empty()) {
+      Args a(ndebug_args);
+      dev.download_state();
+      const std::string f = write_dump_neighbor_debug(sys, export_neighbor_rows(dev), a.n(), a.v(), 0);
+      if (!f.empty()) std::printf("neighbor debug written to %s\n", f.c_str());
+    }
+    auto snapshot = [&](long step) {
+      bool due = false;
+      for (auto& d : dumps) due = due || step % d->nevery == 0;
+      if (!due) return;
+      dev.download_state();
+      dev.download_forces();
+      for (auto& d : dumps)
+        if (step % d->nevery == 0) d->write(sys, step);
+    };
+    snapshot(0);
+    // run in chunks that end on every dump step; thermo rows keep their own
+    // cadence, the rows the chunk ends add are dropped
+    long chunk = steps;
+    for (auto& d : dumps) chunk = std::min<long>(chunk, d->nevery);
+    if (chunk <= 0) chunk = steps;
+    std::vector<ThermoRow> rows;
+    long done = 0;
+    while (done < steps || rows.empty()) {
+      const int n = (int)std::min<long>(chunk, steps - done);
+      auto part = loop == "host" ? verlet.run(n, thermo) : verlet.run_resident(n, dt, thermo);
+      for (size_t k = rows.empty() ? 0 : 1; k < part.size(); k++) {
+        const long st = part[k].step;
+        if (k == 0 || st == steps || (thermo > 0 && st % thermo == 0)) rows.push_back(part[k]);
+      }
+      if (n == 0) break;
+      done += n;
+      snapshot(done);
+    }
     std::printf("Step PotEng KinEng TotEng Press\n");
     for (const auto& r : rows) std::printf("%ld %.15g %.15g %.15g %.15g\n", r.step, r.pe, r.ke, r.etotal, r.press);
     std::printf("Neighbor list builds = %d\nDangerous builds = %d\n", neigh.ncalls(), neigh.ndanger());

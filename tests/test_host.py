@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from cac_amd import decks
+from tests import brute
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RUNNER = os.path.join(ROOT, "cac_amd", "cac_run")
@@ -79,3 +80,160 @@ def test_host_layer_run_matches_oracle(tmp_path, loop):
     assert np.array_equal(rows[:, 0], th[:, 0])
     assert np.allclose(rows[:, 1:4], th[:, 1:4], rtol=1e-10, atol=1e-9)
     assert builds == o.neigh_stats()[0]
+
+
+# ------------------------------------------------------------------ output taps
+
+def read_dump(path):
+    """One snapshot of a LAMMPS-style text dump: (timestep, box[3,2], columns, rows[n, ncol])."""
+    lines = open(path).read().splitlines()
+    assert lines[0] == "ITEM: TIMESTEP" and lines[2] == "ITEM: NUMBER OF ATOMS"
+    n = int(lines[3])
+    assert lines[4].startswith("ITEM: BOX BOUNDS")
+    box = np.array([[float(v) for v in lines[5 + d].split()] for d in range(3)])
+    assert lines[8].startswith("ITEM: ATOMS ")
+    cols = lines[8].split()[2:]
+    rows = np.array([[float(v) for v in ln.split()] for ln in lines[9:9 + n]]).reshape(n, len(cols))
+    return int(lines[1]), box, cols, rows
+
+
+DUMP_COLS = "id type x y z vx vy vz fx eid igauss inode iucell"
+
+
+def dump_from_data(tmp_path, deck, fmt, cols=DUMP_COLS, extra="precision high average yes"):
+    data = tmp_path / "deck.data"
+    decks.write_data(str(data), deck)
+    out = tmp_path / "snap.*.atom"
+    run(data, "--parse-only", "--dump", f"1 all custom 1 {out} format {fmt} {extra} {cols}")
+    return read_dump(str(tmp_path / "snap.0.atom"))
+
+
+def test_dump_custom_node_and_gauss_formats(tmp_path):
+    """dump custom, formats lammps/node and lammps/gauss/point: atoms then each
+    element's 8 points, ids after the largest atom id, positions and velocities
+    exact at 17 digits (precision high)."""
+    d = decks.cac_eam(box_cells=(2, 2, 2), core_cells=4, ncell=3, jitter=0.03, T=300.0, seed=3)
+    A, E = d.natoms, d.nelements
+    corners = brute.corner_ucells(d.ncell)
+    for fmt in ("lammps/node", "lammps/gauss/point"):
+        step, box, cols, r = dump_from_data(tmp_path, d, fmt)
+        assert step == 0 and cols == DUMP_COLS.split()
+        assert np.array_equal(box, np.stack([d.boxlo, d.boxhi], axis=1))
+        assert r.shape[0] == A + 8 * E
+        assert np.array_equal(r[:A, 0], np.arange(1, A + 1))
+        assert np.array_equal(r[:A, 2:5], d.x) and np.array_equal(r[:A, 5:8], d.v)
+        assert np.all(r[:A, 9:] == -1)
+        el = r[A:].reshape(E, 8, -1)
+        assert np.array_equal(el[:, :, 0], A + 1 + np.arange(8 * E).reshape(E, 8))
+        assert np.array_equal(el[:, :, 9], np.repeat(np.arange(1, E + 1)[:, None], 8, axis=1))
+        N = brute.shape_hex(brute.ucell_natural(d.ncell))
+        node_of_gauss = np.array([int(np.argmax(N[u])) for u in corners])
+        gauss_of_node = np.argsort(node_of_gauss)
+        if fmt == "lammps/node":
+            node, gauss = np.arange(8), gauss_of_node
+        else:
+            node, gauss = node_of_gauss, np.arange(8)
+        assert np.array_equal(el[:, :, 2:5], d.nodex[:, node]) and np.array_equal(el[:, :, 5:8], d.nodev[:, node])
+        assert np.array_equal(el[0, :, 10], gauss) and np.array_equal(el[0, :, 11], node)
+        assert np.array_equal(el[0, :, 12], corners[gauss])
+    assert not np.any(r[:, 8])  # no forces in a data file
+
+
+def test_dump_custom_full_map_surface_center(tmp_path):
+    """lammps/full/map interpolates every virtual atom; lammps/surface keeps the
+    outer unit cells; lammps/center writes one line per element at its centre."""
+    d = decks.cac_eam(box_cells=(2, 2, 2), core_cells=4, ncell=5, jitter=0.03, T=300.0, seed=4)
+    A, E, nu = d.natoms, d.nelements, d.nucell
+    pos, kind, owner, ucell = brute.all_sites(d)
+    _, _, cols, r = dump_from_data(tmp_path, d, "lammps/full/map")
+    assert r.shape[0] == A + nu * E
+    assert np.allclose(r[A:, 2:5], pos[A:], rtol=0, atol=1e-12)
+    assert np.array_equal(r[A:, 12], ucell[A:]) and np.array_equal(r[A:, 0], A + 1 + np.arange(nu * E))
+    _, _, _, rs = dump_from_data(tmp_path, d, "lammps/surface")
+    assert rs.shape[0] == A + E * (nu - 3 ** 3)
+    _, _, _, rc = dump_from_data(tmp_path, d, "lammps/center", cols="id type x y z eid iucell")
+    assert rc.shape[0] == A + E
+    assert np.allclose(rc[A:, 2:5], d.nodex.mean(axis=1), rtol=0, atol=1e-12)
+    assert np.all(rc[A:, 1] == 0) and np.all(rc[A:, 6] == -1)
+
+
+def test_dump_custom_argument_rules(tmp_path):
+    """The reference's argument grammar: the keyword pair after wrap or
+    precision is skipped; doubles print as %g unless precision high; unknown
+    columns are an error."""
+    d = decks.cac_eam(box_cells=(2, 2, 2), core_cells=4, ncell=3, jitter=0.03, T=300.0, seed=5)
+    _, _, cols, r = dump_from_data(tmp_path, d, "lammps/node", cols="id x", extra="wrap yes type y")
+    assert cols == ["id", "x"]
+    txt = (tmp_path / "snap.0.atom").read_text().splitlines()
+    assert txt[9].split()[1] == f"{d.x[0, 0]:g}"
+    data = tmp_path / "deck.data"
+    r = run(data, "--parse-only", "--dump", "1 all custom 1 f.atom id nope", check=False)
+    assert r.returncode == 1 and "unknown column keyword" in r.stderr
+    r = run(data, "--parse-only", "--dump", "1 all custom 1 f.atom id ix", check=False)
+    assert r.returncode == 1 and "image" in r.stderr
+
+
+@pytest.mark.gpu
+def test_dump_forces_match_oracle(tmp_path):
+    """dump custom fx fy fz at step 0 (atoms, and nodal forces in lammps/node)
+    against the oracle's forces."""
+    from oracle import oracle as O
+    d = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=300.0, seed=22)
+    data, setfl = tmp_path / "deck.data", tmp_path / "cu.setfl"
+    decks.write_data(str(data), d)
+    decks.write_setfl(str(setfl), TAB)
+    run(data, "--eam", setfl, "--skin", 1.0, "--steps", 0, "--dump",
+        f"1 all custom 5 {tmp_path}/f.*.atom format lammps/node precision high average yes id x y z fx fy fz")
+    _, _, cols, r = read_dump(str(tmp_path / "f.0.atom"))
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=decks.read_setfl(str(setfl)))
+    o.setup()
+    _, _, f = o.atoms()
+    nodef = o.elements()["nodef"].reshape(-1, 3)
+    want = np.concatenate([f, nodef])
+    assert np.max(np.abs(r[:, 4:7] - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+@pytest.mark.gpu
+def test_neighbor_debug_rows_match_oracle(tmp_path):
+    """write_dump_neighbor_debug: list lengths of every row, and one atom row and
+    one Gauss row entry by entry, against the oracle's lists as full sets."""
+    from oracle import oracle as O
+    d = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=300.0, seed=23)
+    data, setfl = tmp_path / "deck.data", tmp_path / "cu.setfl"
+    decks.write_data(str(data), d)
+    decks.write_setfl(str(setfl), TAB)
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=decks.read_setfl(str(setfl)))
+    o.setup()
+    p, k, ow, u, img = o.half_list()
+    sets = O.canonical_full_sets(d, p, k, ow, u, img, brute.corner_ucells(d.ncell))
+    A, E = d.natoms, d.nelements
+    counts = np.bincount(sets[:, 0], minlength=A + 8 * E)
+    run(data, "--eam", setfl, "--skin", 1.0, "--steps", 0, "--neighbor-debug", f"{tmp_path}/nl")
+    _, _, cols, r = read_dump(str(tmp_path / "nl_proc_0.dat"))
+    assert cols == "id type x y z tag numneigh igcell ibasis".split()
+    assert np.array_equal(r[:, 6], counts)
+    assert np.array_equal(r[:A, 5], np.arange(1, A + 1)) and np.all(r[:A, 7] == -1)
+    assert np.array_equal(r[A:, 7], np.tile(np.arange(8), E))
+    L = d.boxhi - d.boxlo
+    x_o = o.atoms()[0]
+    N = brute.shape_hex(brute.ucell_natural(d.ncell))
+    vpos = np.einsum("uk,ekd->eud", N, o.elements()["nodex"])
+    for args, point in ((f"{tmp_path}/one atom 7", 6), (f"{tmp_path}/two gauss 5 0 3", A + 8 * 4 + 3)):
+        run(data, "--eam", setfl, "--skin", 1.0, "--steps", 0, "--neighbor-debug", args)
+        name = args.split()[0] + "_proc_0"
+        _, _, cols, r = read_dump(name)
+        assert cols == "id type x y z tag iucell ibasis".split()
+        mine = r[1:]
+        ref = sets[sets[:, 0] == point]
+        assert mine.shape[0] == ref.shape[0]
+        want_type = np.where(ref[:, 1] == 0, 2, 3)
+        want_tag = ref[:, 2] + 1
+        want_uc = np.where(ref[:, 1] == 0, -1, ref[:, 3])
+        want_x = np.where(ref[:, 1:2] == 0, x_o[np.minimum(ref[:, 2], A - 1)],
+                          vpos[np.minimum(ref[:, 2], E - 1), ref[:, 3]]) + ref[:, 4:7] * L
+        left = list(range(ref.shape[0]))
+        for row in mine:  # each written neighbor matches one oracle entry: kind, tag, unit cell, position
+            hit = [q for q in left if (want_type[q], want_tag[q], want_uc[q]) == tuple(row[[1, 5, 6]])
+                   and np.max(np.abs(want_x[q] - row[2:5])) < 1e-3]
+            assert hit, row
+            left.remove(hit[0])
