@@ -31,9 +31,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 WORKLOADS = {
-    # name: (box cells, core cells, ncell, description)
-    "cac_eam_x8": ((14, 16, 16), 351, 9, "255,879 atoms + 13,985 Hex(9^3) elements, EAM, mixed vs FP64"),
-    "cac_eam": ((7, 8, 8), 44, 9, "32,076 atoms + 1,748 Hex(9^3) elements, EAM, mixed vs FP64"),
+    # name: (deck builder, description)
+    "cac_eam_x8": (lambda d: d.cac_eam(box_cells=(14, 16, 16), core_cells=351, ncell=9, jitter=0.02, T=300.0,
+                                       seed=2024),
+                   "255,879 atoms + 13,985 Hex(9^3) elements, EAM, mixed vs FP64"),
+    "cac_eam": (lambda d: d.cac_eam(box_cells=(7, 8, 8), core_cells=44, ncell=9, jitter=0.02, T=300.0, seed=2024),
+                "32,076 atoms + 1,748 Hex(9^3) elements, EAM, mixed vs FP64"),
+    "cac_dislocation": (lambda d: d.cac_dislocation(),
+                        "2,000,493 atoms around an edge dislocation dipole + 40,097 Hex(11^3) elements, "
+                        "55.4 M represented sites, EAM, mixed vs FP64"),
 }
 SKIN, DT = 2.0, 0.001
 HBM_PEAK_GBS = 8000.0
@@ -149,9 +155,9 @@ def main():
     import cac_amd
     from cac_amd import decks
 
-    box, core, nc, desc = WORKLOADS[args.workload]
+    build_deck, desc = WORKLOADS[args.workload]
     tab = decks.cu_eam_tables()
-    deck = decks.cac_eam(box_cells=box, core_cells=core, ncell=nc, jitter=0.02, T=300.0, seed=2024)
+    deck = build_deck(decks)
     P = deck.atom_equivalents
 
     if args.profile_run:

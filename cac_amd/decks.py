@@ -248,16 +248,21 @@ def superlattice_cells(a: float, ncell: int, box_cells) -> np.ndarray:
     return cells
 
 
-def cac_geometry(a: float, ncell: int, box_cells, core_cells: int):
+def cac_geometry(a: float, ncell: int, box_cells, core_cells: int, core_axis=None):
     """Void-free CAC geometry: the periodic box ncell*a*(nx, ny, nz) is tiled by
     the element superlattice; the ``core_cells`` cells nearest the box centre
-    are atomistic (their ncell^3 lattice sites become atoms) and every other cell
-    is an element.  Returns (box lengths, atom positions, element cells, nodes)."""
+    (nearest the box's central line along ``core_axis`` when one is given: a
+    cylindrical core) are atomistic (their ncell^3 lattice sites become atoms)
+    and every other cell is an element.  Returns (box lengths, atom positions,
+    element cells, nodes)."""
     b = primitive_vectors(a)
     L = ncell * a * np.asarray(box_cells, dtype=np.float64)
     cells = superlattice_cells(a, ncell, box_cells)
     centre = (ncell * cells + 0.5 * (ncell - 1)).astype(np.float64) @ b
-    d = np.linalg.norm(centre - 0.5 * L, axis=1)
+    off = centre - 0.5 * L
+    if core_axis is not None:
+        off[:, core_axis] = 0.0
+    d = np.linalg.norm(off, axis=1)
     idx = np.arange(cells.shape[0])
     order = idx[np.lexsort((idx, np.round(d, 9)))]
     core = np.sort(order[:core_cells])
@@ -367,6 +372,72 @@ def cac_eam(box_cells=(7, 8, 8), core_cells: int = 44, ncell: int = 9, jitter: f
                 atype=np.ones(x.shape[0], dtype=np.int32), ncell=(ncell, ncell, ncell), nodex=nodes,
                 nodev=nv, etype=np.ones(E, dtype=np.int32), ctype=np.ones(E, dtype=np.int32),
                 name=f"cac_eam_box{'x'.join(map(str, box_cells))}_core{core_cells}_n{ncell}")
+
+
+def edge_dislocation_field(x1, x2, b: float, nu: float, theta: float):
+    """Isotropic Volterra displacement (u1, u2) of a straight edge dislocation
+    at the origin of the (x1, x2) plane with Burgers vector b along the
+    direction at angle theta from x1 (displace.cpp:956-974): in the frame
+    rotated by theta, u1 = b/2pi [atan2(r2, r1) + pi + r1 r2 / (2 (1 - nu) r^2)],
+    u2 = -b / (8 pi (1 - nu)) [(1 - 2 nu) ln r^2 + (r1^2 - r2^2) / r^2], then
+    rotated back.  The cut (the jump of b in u1) lies on the half-plane r2 = 0,
+    r1 < 0."""
+    c, s = np.cos(theta), np.sin(theta)
+    r1 = x1 * c + x2 * s
+    r2 = -x1 * s + x2 * c
+    rsq = r1 * r1 + r2 * r2
+    if np.any(rsq < 1.0e-4):
+        raise ValueError("a site is too close to the dislocation line")
+    u1 = b / (2.0 * np.pi) * ((np.arctan2(r2, r1) + np.pi) + r1 * r2 / (2.0 * (1.0 - nu) * rsq))
+    u2 = -b / (8.0 * np.pi * (1.0 - nu)) * ((1.0 - 2.0 * nu) * np.log(rsq) + (r1 * r1 - r2 * r2) / rsq)
+    return u1 * c - u2 * s, u1 * s + u2 * c
+
+
+def cac_dislocation(box_cells=(20, 20, 26), core_cells: int = 1503, ncell: int = 11, T: float = 300.0,
+                    seed: int = 2025, a: float = 3.615, mass: float = 63.546, nu: float = 0.34,
+                    separation: int = 24) -> Deck:
+    """Large single system: FCC Cu holding edge dislocations with b = a/2 [110]
+    and lines along z, inserted by the Volterra field of
+    :func:`edge_dislocation_field` (nu = 0.34) on atoms and element nodes.
+
+    A single Volterra dislocation in a periodic box leaves its cut open where
+    it meets the box boundary (nodes of neighbouring elements 0.7 A apart on
+    this box), so the periodic share holds it as a dipole: the dislocation and
+    one of opposite sign ``separation`` Burgers vectors away along b.  Their
+    cuts cancel outside the strip between the lines, which lies in the
+    atomistic core, and the far field decays.  The lines sit at (a/8, 3a/8)
+    from a lattice row in (x, y), so that no row lies on a line or on the cut
+    plane (a row on the cut would slip by b onto its neighbour).
+
+    The atomistic core is the cylinder of ``core_cells`` superlattice cells
+    nearest the box's central z line; the rest are Hex elements of ncell^3
+    sites.  Defaults: 1,503 cells of 11^3 sites = 2,000,493 atoms in 40,097
+    elements, 55.4 M represented sites, box 795 x 795 x 1034 A.  Velocities
+    Maxwell at T with zero net momentum."""
+    rng = np.random.default_rng(seed)
+    L, x, cells, nodes = cac_geometry(a, ncell, box_cells, core_cells, core_axis=2)
+    bvec, theta = a / np.sqrt(2.0), 0.25 * np.pi
+    step = 0.5 * a * np.array([1.0, 1.0])  # one Burgers vector along [110]
+    c1 = 0.5 * L[:2] + a * np.array([0.125, 0.375]) - 0.5 * separation * step
+    c2 = c1 + separation * step
+
+    def displaced(p):
+        u1, u2 = edge_dislocation_field(p[..., 0] - c1[0], p[..., 1] - c1[1], bvec, nu, theta)
+        w1, w2 = edge_dislocation_field(p[..., 0] - c2[0], p[..., 1] - c2[1], -bvec, nu, theta)
+        q = p.copy()
+        q[..., 0] += u1 + w1
+        q[..., 1] += u2 + w2
+        return q
+
+    x = displaced(x)
+    x = x - L * np.floor(x / L)
+    nodes = displaced(nodes)
+    v, nv = maxwell(rng, x.shape[0], nodes.shape[0], mass, T, BOLTZ_METAL, MVV2E_METAL, ncell ** 3)
+    E = nodes.shape[0]
+    return Deck(units="metal", boxlo=np.zeros(3), boxhi=L.copy(), masses=np.array([mass]), x=x, v=v,
+                atype=np.ones(x.shape[0], dtype=np.int32), ncell=(ncell, ncell, ncell), nodex=nodes,
+                nodev=nv, etype=np.ones(E, dtype=np.int32), ctype=np.ones(E, dtype=np.int32),
+                name=f"cac_dislocation_box{'x'.join(map(str, box_cells))}_core{core_cells}_n{ncell}")
 
 
 def wrap_atoms(deck: Deck) -> Deck:

@@ -96,6 +96,50 @@ def test_cac_eam_sets_forces(nc, box, core, skin):
     assert rel(ar_g, ar_o) < 1e-13 and rel(nr_g, nr_o) < 1e-13
 
 
+def test_dislocation_deck_sets_forces_trajectory():
+    """The large-system geometry (cylindrical core around an edge dislocation
+    dipole, Hex elements) at a size the oracle finishes: exact sets, FP64
+    forces within 1e-12, 20 steps within 1e-10, mixed forces within 1e-5."""
+    import cac_amd
+    deck = decks.cac_dislocation(box_cells=(4, 4, 4), core_cells=40, ncell=3, separation=4)
+    o, g = setup_pair(deck, "eam", 2.0)
+    assert np.array_equal(g.full_sets(), oracle_sets(o, deck))
+    check_forces(o, g, deck)
+    th_o = o.run(20, 0.001, thermo_every=10)
+    th_g = g.run(20, 0.001, thermo_every=10)
+    assert np.allclose(th_g[:, 1:4], th_o[:, 1:4], rtol=1e-10, atol=1e-9)
+    gm = gpu_model(deck, "eam", 2.0, precision=cac_amd.MIXED)
+    gm.setup()
+    g64 = gpu_model(deck, "eam", 2.0)
+    g64.setup()
+    f64 = np.concatenate([g64.atoms()[2], g64.elements()["gaussf"].reshape(-1, 3)])
+    f32 = np.concatenate([gm.atoms()[2], gm.elements()["gaussf"].reshape(-1, 3)])
+    assert rel(f32, f64) < 1e-5
+
+
+def test_dislocation_full_size_mixed_vs_fp64():
+    """The full large system (2,000,493 atoms + 40,097 elements of 11^3 sites,
+    2,321,269 points): mixed forces within 1e-5 of FP64, list statistics, and
+    50 mixed steps that run (lists grow as needed) and stay finite."""
+    import cac_amd
+    deck = decks.cac_dislocation()
+    res = {}
+    for prec in (cac_amd.FP64, cac_amd.MIXED):
+        g = gpu_model(deck, "eam", 2.0, precision=prec)
+        g.setup()
+        res[prec] = np.concatenate([g.atoms()[2], g.elements()["gaussf"].reshape(-1, 3)])
+        info = g.neigh_info()
+        assert info["rows"] == deck.atom_equivalents
+        assert 100 < info["entries"] / info["rows"] < 200, info
+        if prec == cac_amd.MIXED:
+            th = g.run(50, 0.001, thermo_every=25)
+            assert np.all(np.isfinite(th))
+            x, _, f = g.atoms()
+            assert np.all(np.isfinite(x)) and np.all(np.isfinite(f))
+        g.close()
+    assert rel(res[cac_amd.MIXED], res[cac_amd.FP64]) < 1e-5
+
+
 def test_cac_lj_sets_forces():
     rng = np.random.default_rng(1)
     L, x, cells, nodes = decks.cac_geometry(1.55, 3, (4, 4, 4), 20)
