@@ -91,6 +91,43 @@ def nve_drift(cac_amd, deck, tab, precision, steps, every=100):
             "e0": float(e[0]), "e_end": float(e[-1])}
 
 
+def serve_run(cac_amd, decks, tab, n, steps, rate, slots, seed=77):
+    """The serving workload: ``n`` replica requests (sizes log-uniform 2k-200k
+    atom-eq, core/coarse 0.05-0.3, 100-600 K, +-2 % uniaxial strain) arriving
+    as a Poisson stream at ``rate`` per second, each running ``steps`` mixed
+    steps on the resident server.  Requests are submitted at their arrival
+    times; decks are built before the clock starts."""
+    import numpy as np
+    specs = decks.serving_stream(n=n, seed=seed, rate=rate)
+    dks = [decks.serving_deck(sp) for sp in specs]
+    srv = cac_amd.Server(slots=slots, eam_tables=tab)
+    t0 = time.perf_counter()
+    ids = []
+    for sp, d in zip(specs, dks):
+        wait = sp["arrival"] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(srv.submit(d, steps, DT, skin=SKIN, precision=cac_amd.MIXED))
+    srv.drain()
+    wall = time.perf_counter() - t0
+    res = [srv.result(i) for i in ids]
+    st = srv.stats()
+    err = srv.L.cac_hip_server_last_error(srv.h).decode()
+    srv.close()
+    q = np.array([r["t_admit"] - r["t_submit"] for r in res])
+    mean_step = np.array([r["step_ms_mean"] for r in res])
+    ae = np.array([r["atom_equivalents"] for r in res], dtype=float)
+    return {"replicas": n, "steps_each": steps, "arrival_rate_per_s": rate, "slots": slots,
+            "failed": int(sum(r["status"] != 0 for r in res)), "last_error": err or None,
+            "atom_eq_mean": float(ae.mean()), "atom_eq_range": [float(ae.min()), float(ae.max())],
+            "step_ms_p50": st["p50_ms"], "step_ms_p99": st["p99_ms"],
+            "replica_mean_step_ms_p50": float(np.percentile(mean_step, 50)),
+            "replica_mean_step_ms_p99": float(np.percentile(mean_step, 99)),
+            "queue_wait_s_p50": float(np.percentile(q, 50)), "queue_wait_s_p99": float(np.percentile(q, 99)),
+            "atom_steps_per_s": st["atom_steps_per_s"], "wall_s": wall, "peak_active": st["peak_active"],
+            "peak_hbm_gib_estimate": st["peak_hbm_bytes"] / 2 ** 30}
+
+
 def force_bytes(P, inner_mean, live_sites):
     """Algorithmic HBM bytes of one mixed EAM force launch.
 
@@ -142,6 +179,14 @@ def main():
                     help="prune margin delta of the inner list (A); < 0 takes 0.35 * skin")
     ap.add_argument("--drift-steps", type=int, default=10000,
                     help="NVE steps of the FP64-vs-mixed drift comparison on the 32,000-atom EAM block (0: skip)")
+    ap.add_argument("--serve-replicas", type=int, default=512,
+                    help="replicas of the serving run (0: skip)")
+    ap.add_argument("--serve-steps", type=int, default=250,
+                    help="steps per replica: CAC with 2x2x2 Gauss points on 9^3 elements is unstable past "
+                         "about 450 steps (element deformed more than 2x, in the oracle as on the GPU)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate (replicas per second)")
+    ap.add_argument("--serve-slots", type=int, default=32)
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--profile-run", action="store_true",
                     help="mixed model only, --steps steps, then a 1 GiB copy (program of the rocprofv3 summaries)")
     args = ap.parse_args()
@@ -157,6 +202,10 @@ def main():
 
     build_deck, desc = WORKLOADS[args.workload]
     tab = decks.cu_eam_tables()
+    if args.serve_only:
+        print(json.dumps({"serving": serve_run(cac_amd, decks, tab, args.serve_replicas, args.serve_steps,
+                                               args.serve_rate, args.serve_slots)}))
+        return
     deck = build_deck(decks)
     P = deck.atom_equivalents
 
@@ -232,6 +281,11 @@ def main():
                  "ratio_mixed_over_fp64": abs(dmx["drift_eV_per_atom_ps"]) / max(abs(d64["drift_eV_per_atom_ps"]),
                                                                                    1e-300)}
 
+    serving = None
+    if args.serve_replicas > 0:
+        serving = serve_run(cac_amd, decks, tab, args.serve_replicas, args.serve_steps, args.serve_rate,
+                            args.serve_slots)
+
     cpu = None
     if not args.no_cpu_baseline:
         t_setup, t_run = cpu_baseline(deck, tab, args.cpu_steps)
@@ -269,6 +323,7 @@ def main():
                      "algorithmic_bytes": bytes_force,
                      "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if tr else None},
         "nve_drift": drift,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

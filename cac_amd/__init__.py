@@ -28,6 +28,23 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
+class _Request(C.Structure):
+    """cac_hip_request of include/cac_hip.h"""
+    _fields_ = [("units", C.c_int), ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("ntypes", C.c_int),
+                ("mass", C.c_void_p), ("ncell", C.c_int * 3), ("natoms", C.c_int), ("x", C.c_void_p),
+                ("v", C.c_void_p), ("type", C.c_void_p), ("nelements", C.c_int), ("ctype", C.c_void_p),
+                ("nodex", C.c_void_p), ("nodev", C.c_void_p), ("skin", C.c_double), ("precision", C.c_int),
+                ("nsteps", C.c_int), ("dt", C.c_double), ("keep_state", C.c_int)]
+
+
+class _Result(C.Structure):
+    """cac_hip_result of include/cac_hip.h"""
+    _fields_ = [("status", C.c_int), ("nsteps", C.c_int), ("atom_equivalents", C.c_long),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("step_ms_mean", C.c_double), ("pe", C.c_double), ("ke", C.c_double), ("etotal", C.c_double),
+                ("builds", C.c_long)]
+
+
 class CacHipError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cac_hip error {code} ({ERRORS.get(code, '?')}): {msg}")
@@ -88,6 +105,16 @@ def lib():
             "cac_hip_set_inner_skin": [vp, d],
             "cac_hip_sync_positions": [vp, vp, vp],
             "cac_hip_download_forces": [vp, vp, vp, vp],
+            "cac_hip_server_create": [C.POINTER(vp), i, i, d],
+            "cac_hip_server_destroy": [vp],
+            "cac_hip_server_pair_eam": [vp, i, d, i, d, d, _dp, _dp, _dp],
+            "cac_hip_server_pair_lj": [vp, d, d, d],
+            "cac_hip_server_submit": [vp, C.POINTER(_Request), C.POINTER(C.c_long)],
+            "cac_hip_server_drain": [vp],
+            "cac_hip_server_result": [vp, C.c_long, C.POINTER(_Result)],
+            "cac_hip_server_state": [vp, C.c_long, vp, vp],
+            "cac_hip_server_stats": [vp, C.POINTER(d), C.POINTER(d), C.POINTER(C.c_long), C.POINTER(d),
+                                     C.POINTER(i), C.POINTER(d)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -95,6 +122,8 @@ def lib():
             fn.restype = C.c_int
         L.cac_hip_last_error.argtypes = [vp]
         L.cac_hip_last_error.restype = C.c_char_p
+        L.cac_hip_server_last_error.argtypes = [vp]
+        L.cac_hip_server_last_error.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -272,3 +301,88 @@ class Model:
         a, n = C.c_double(), C.c_long()
         self._chk(self.L.cac_hip_kernel_time(self.h, name.encode(), C.byref(a), C.byref(n)))
         return a.value, n.value
+
+
+class Server:
+    """The resident serving loop (cac_hip_server): independent replicas
+    admitted by slot and HBM budget and stepped round-robin on one GPU."""
+
+    def __init__(self, slots: int = 16, hbm_budget_bytes: float = 0.0, pair: str = "eam", eam_tables=None,
+                 lj=None, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.cac_hip_server_create(C.byref(h), device, slots, hbm_budget_bytes)
+        if rc:
+            raise CacHipError(rc, "cac_hip_server_create failed")
+        self.h = h
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        if pair == "eam":
+            tb = eam_tables
+            self._chk(self.L.cac_hip_server_pair_eam(h, tb.nrho, tb.drho, tb.nr, tb.dr, tb.cut, f64(tb.frho),
+                                                     f64(tb.rhor), f64(tb.z2r)))
+        elif pair == "lj":
+            p = lj or dict(eps=1.0, sigma=1.0, cut=2.5)
+            self._chk(self.L.cac_hip_server_pair_lj(h, p["eps"], p["sigma"], p["cut"]))
+        else:
+            raise ValueError(pair)
+
+    def _chk(self, rc):
+        if rc:
+            raise CacHipError(rc, self.L.cac_hip_server_last_error(self.h).decode())
+
+    def submit(self, deck, nsteps: int, dt: float, skin: float = 2.0, precision: int = MIXED,
+               keep_state: bool = False) -> int:
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        keep = [f64(deck.masses), f64(deck.x), f64(deck.v), i32(deck.atype)]
+        r = _Request()
+        r.units = UNITS[deck.units]
+        r.lo[:] = list(map(float, deck.boxlo))
+        r.hi[:] = list(map(float, deck.boxhi))
+        r.ntypes = len(deck.masses)
+        r.mass = _p(keep[0])
+        r.natoms = deck.natoms
+        r.x, r.v, r.type = _p(keep[1]), _p(keep[2]), _p(keep[3])
+        r.nelements = deck.nelements
+        if deck.nelements:
+            keep += [i32(deck.ctype), f64(deck.nodex), f64(deck.nodev)]
+            r.ctype, r.nodex, r.nodev = _p(keep[4]), _p(keep[5]), _p(keep[6])
+            r.ncell[:] = list(deck.ncell)
+        r.skin, r.precision, r.nsteps, r.dt, r.keep_state = skin, precision, nsteps, dt, int(keep_state)
+        rid = C.c_long()
+        self._chk(self.L.cac_hip_server_submit(self.h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def drain(self):
+        self._chk(self.L.cac_hip_server_drain(self.h))
+
+    def result(self, rid: int) -> dict:
+        r = _Result()
+        self._chk(self.L.cac_hip_server_result(self.h, rid, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in _Result._fields_}
+
+    def state(self, rid: int, deck):
+        x = np.zeros((deck.natoms, 3))
+        nx = np.zeros((deck.nelements, 8, 3))
+        self._chk(self.L.cac_hip_server_state(self.h, rid, _p(x) if deck.natoms else None,
+                                              _p(nx) if deck.nelements else None))
+        return x, nx
+
+    def stats(self) -> dict:
+        p50, p99, rate, hbm = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        n, peak = C.c_long(), C.c_int()
+        self._chk(self.L.cac_hip_server_stats(self.h, C.byref(p50), C.byref(p99), C.byref(n), C.byref(rate),
+                                              C.byref(peak), C.byref(hbm)))
+        return dict(p50_ms=p50.value, p99_ms=p99.value, steps=n.value, atom_steps_per_s=rate.value,
+                    peak_active=peak.value, peak_hbm_bytes=hbm.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cac_hip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -16,10 +16,6 @@
 
 using namespace cac;
 
-struct cac_hip_ctx {
-  Ctx c;
-};
-
 namespace cac {
 
 Ctx::~Ctx() {
@@ -50,6 +46,80 @@ ScopedTimer::~ScopedTimer() {
   Timer& t = c.timers[name];
   t.sum_us += 1000.0 * ms;
   t.count++;
+}
+
+// One velocity-Verlet step (verlet.cpp:218-360), in two halves so that a
+// caller can queue the first half of several models before it waits on any.
+// step_issue advances the step counter and runs Neighbor::decide: when a
+// rebuild is due without a check it does the whole step and returns false.
+// Otherwise it queues kick + drift + both displacement checks and the site
+// refresh as if no rebuild were due (the refresh returns at once if the check
+// asks for a rebuild, and refreshes every site if it asks for a re-prune), and
+// returns true: step_resolve then waits for the check, rebuilds or re-prunes
+// if asked, and queues the force passes and the closing half kick.
+bool step_issue(Ctx& c, double dt, int ev) {
+  c.ntimestep++;
+  const int N = (int)c.ntimestep;
+  c.ago++;
+  const bool due = c.ago >= c.delay && c.ago % c.every == 0;
+  if (due && !c.check) {
+    nve_initial(c, dt);
+    build_neighbor(c);
+    pair_passes(c, ev, ev, Gate());
+    step_end(c, dt, Gate());
+    return false;
+  }
+  const Gate g{c.d_steps.get(), N};
+  step_begin(c, dt, N, due);
+  CAC_HIP(hipEventRecord(c.ev_check, c.stream));
+  site_refresh(c, false, g);
+  return true;
+}
+
+void step_resolve(Ctx& c, double dt, int ev) {
+  const int N = (int)c.ntimestep;
+  CAC_HIP(hipEventSynchronize(c.ev_check));
+  if (c.h_steps[0] == N) {
+    if (c.ago == std::max(c.every, c.delay)) c.ndanger++;
+    build_neighbor(c);
+  } else if (c.h_steps[1] == N) {
+    build_inner(c);
+  }
+  pair_passes(c, ev, ev, Gate());
+  step_end(c, dt, Gate());
+}
+
+// A context about to take a new model: box, types, element type, staged
+// arrays, run counters and status words go back to their initial state; the
+// pair style, the neighbor settings and every device buffer are kept, so a
+// model of at most the same size loads without a device allocation.
+void reset_model(Ctx& c) {
+  c.box_set = false;
+  c.ntypes = 0;
+  c.etype_set = false;
+  c.st_ax.clear();
+  c.st_av.clear();
+  c.st_nx.clear();
+  c.st_nv.clear();
+  c.st_at.clear();
+  c.st_ct.clear();
+  c.A = c.E = c.P = 0;
+  c.model_set = false;
+  c.built = false;
+  c.ago = c.ncalls = c.ndanger = 0;
+  c.ntimestep = 0;
+  c.inner_builds = 0;
+  c.inner_mode = -1;
+  c.inner_due = 0;
+  c.x_dirty = false;
+  c.entries = -1;
+  c.eng_vdwl = 0.0;
+  for (double& v : c.virial) v = 0.0;
+  c.timers.clear();
+  CAC_HIP(hipMemsetAsync(c.d_flags.get(), 0, sizeof(ErrorFlags), c.stream));
+  CAC_HIP(hipMemsetAsync(c.d_steps.get(), 0xFF, sizeof(int) * 2, c.stream));
+  CAC_HIP(hipStreamSynchronize(c.stream));
+  c.h_steps[0] = c.h_steps[1] = -1;
 }
 
 }  // namespace cac
@@ -724,38 +794,9 @@ int cac_hip_run(cac_hip_ctx* h, int nsteps, double dt, int thermo_every, double*
     };
     if (th) record();
     for (int s = 0; s < nsteps; s++) {
-      c.ntimestep++;
-      const int N = (int)c.ntimestep;
       const bool last = s == nsteps - 1;
-      const int ev = th && ((thermo_every > 0 && c.ntimestep % thermo_every == 0) || last) ? 1 : 0;
-      // Neighbor::decide: the reference's trigger when due, and every step the
-      // inner list's own displacement test
-      c.ago++;
-      const bool due = c.ago >= c.delay && c.ago % c.every == 0;
-      if (due && !c.check) {  // rebuild without a check
-        nve_initial(c, dt);
-        build_neighbor(c);
-        pair_passes(c, ev, ev, Gate());
-        step_end(c, dt, Gate());
-      } else {
-        // queue the site refresh as if no rebuild were due (it returns at once
-        // if the check in step_begin asks for one, and refreshes every site if
-        // the check asks for a re-prune); the host waits for the check while
-        // the refresh runs, then queues the rest of the step
-        const Gate g{c.d_steps.get(), N};
-        step_begin(c, dt, N, due);
-        CAC_HIP(hipEventRecord(c.ev_check, c.stream));
-        site_refresh(c, false, g);
-        CAC_HIP(hipEventSynchronize(c.ev_check));
-        if (c.h_steps[0] == N) {
-          if (c.ago == std::max(c.every, c.delay)) c.ndanger++;
-          build_neighbor(c);
-        } else if (c.h_steps[1] == N) {
-          build_inner(c);
-        }
-        pair_passes(c, ev, ev, Gate());
-        step_end(c, dt, Gate());
-      }
+      const int ev = th && ((thermo_every > 0 && (c.ntimestep + 1) % thermo_every == 0) || last) ? 1 : 0;
+      if (step_issue(c, dt, ev)) step_resolve(c, dt, ev);
       if (ev) pair_sums(c, ev, ev);
       if (ev) record();
     }

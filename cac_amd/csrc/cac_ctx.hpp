@@ -301,6 +301,10 @@ void check_deformation(Ctx& c, bool first);
 void build_neighbor(Ctx& c);  // pbc + sites + cells + full lists
 int check_distance(Ctx& c, bool outer, int* inner);  // 1 when any point moved more than skin/2
 void build_inner(Ctx& c);  // prune the outer list to rc + delta and compact the live sites
+// api.hip: one velocity-Verlet step in two halves (queue, then resolve the check)
+bool step_issue(Ctx& c, double dt, int ev);
+void step_resolve(Ctx& c, double dt, int ev);
+void reset_model(Ctx& c);  // before loading a new model into a used context
 void compact_live(Ctx& c);  // dense live-site records from the full ones (after a prune)
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
                   std::vector<int>& ucell, std::vector<int>& img);
@@ -342,3 +346,8 @@ struct ScopedTimer {  // GPU time between two events on the context stream; nest
 inline int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
 
 }  // namespace cac
+
+// the C ABI's opaque handle (include/cac_hip.h)
+struct cac_hip_ctx {
+  cac::Ctx c;
+};

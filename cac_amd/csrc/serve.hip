@@ -1,0 +1,509 @@
+// serve.hip -- the resident serving loop of include/cac_hip.h (cac_hip_server_*).
+//
+// Independent replicas (whole models, each a Verlet::run of its own,
+// verlet.cpp:218-360) are submitted from any host thread and served by one
+// service thread that owns a fixed set of slots.  A slot is a device context
+// with its own stream; it keeps its buffers from one replica to the next, so
+// serving never frees device memory (a free synchronizes the whole device).
+//
+// Admission: replicas wait in arrival order; the head of the queue is admitted
+// when a slot is free and the HBM it needs, estimated from its size, fits the
+// budget beside what every slot already holds (a slot's holding only grows).
+//
+// Stepping: every round queues the first half of a step (kick, drift, both
+// displacement checks, site refresh) for every active replica, then walks the
+// replicas again, waits for each one's check, rebuilds or re-prunes if asked
+// and queues the force passes and the closing kick.  By the time the service
+// thread comes back to a replica its check has long finished, so the thread
+// rarely waits, and the kernels of different replicas, on different streams,
+// overlap on the device.  A replica's results do not depend on the others:
+// its kernels, launch shapes and summation order are those of a lone run.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/cac_hip.h"
+#include "cac_ctx.hpp"
+
+using namespace cac;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// an owned copy of a submitted request
+struct Request {
+  cac_hip_request r{};
+  std::vector<double> mass, x, v, nodex, nodev;
+  std::vector<int> type, ctype, etype;
+  long id = 0;
+  double t_submit = 0.0;
+  double need = 0.0;  // estimated device bytes
+};
+
+struct Outcome {
+  cac_hip_result r{};
+  std::vector<double> x, nodex;
+  bool done = false;
+};
+
+struct Slot {
+  cac_hip_ctx* h = nullptr;
+  double held = 0.0;  // device bytes this slot's buffers are estimated to hold
+  bool busy = false;
+};
+
+struct Active {
+  std::unique_ptr<Request> req;
+  int slot = -1;
+  int done = 0;
+  double t_admit = 0.0, t_first = 0.0, t_last = 0.0;
+  bool issued = false;
+};
+
+// Device bytes a model needs, from its sizes: per point the two lists
+// (about 230 entries of capacity each at rc + skin = 6.95 A, with headroom) and
+// the point arrays; per represented site the candidate sort arrays; per atom
+// and per element the compacted site records (an element keeps about 270 of
+// its 9^3 sites referenced).
+double need_bytes(const cac_hip_request& q) {
+  const double P = (double)q.natoms + 8.0 * q.nelements;
+  const double nucell = (double)q.ncell[0] * q.ncell[1] * q.ncell[2];
+  const double rep = (double)q.natoms + nucell * q.nelements;
+  const double outer = (double)q.natoms + 270.0 * q.nelements;
+  return 2600.0 * P + 40.0 * rep + 150.0 * outer + (64 << 20);
+}
+
+}  // namespace
+
+struct cac_hip_server {
+  int device = 0;
+  double budget = 0.0;
+  std::string err;
+  std::vector<Slot> slots;
+  bool pair_set = false, submitted = false;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::unique_ptr<Request>> queue;
+  std::map<long, Outcome> out;
+  long next_id = 0, finished = 0;
+  bool stop = false;
+  std::thread th;
+  Clock::time_point t0;
+
+  // statistics, written by the service thread under mu
+  std::vector<float> interval_ms;
+  double peak_hbm = 0.0;
+  int peak_active = 0;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+int set_err(cac_hip_server* s, int code, const std::string& m) {
+  if (s) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->err = m;
+  }
+  return code;
+}
+
+// load a request into a slot's context and set it up (neighbor build + first
+// force); returns 0 or the CAC_HIP_E* code
+int load(Slot& sl, const Request& q, std::string& msg) {
+  cac_hip_ctx* h = sl.h;
+  Ctx& c = h->c;
+  try {
+    reset_model(c);
+  } catch (const HipError& e) {
+    msg = std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where;
+    return CAC_HIP_EHIP;
+  }
+  const cac_hip_request& r = q.r;
+  int rc = 0;
+  auto step = [&](int code) {
+    if (!rc && code) {
+      rc = code;
+      msg = cac_hip_last_error(h);
+    }
+  };
+  if (r.units != c.units) {
+    msg = "a replica's units differ from the server's";
+    return CAC_HIP_EARG;
+  }
+  step(cac_hip_set_box(h, r.lo, r.hi));
+  step(cac_hip_set_masses(h, r.ntypes, r.mass));
+  if (r.nelements > 0) step(cac_hip_add_etype(h, 1, r.ncell[0], r.ncell[1], r.ncell[2], 2, 2, 2));
+  step(cac_hip_upload_atoms(h, r.natoms, r.x, r.v, r.type));
+  step(cac_hip_upload_elements(h, r.nelements, q.etype.data(), r.ctype, r.nodex, r.nodev));
+  step(cac_hip_set_precision(h, r.precision));
+  step(cac_hip_neighbor_settings(h, r.skin, 1, 10, 1, 2000));
+  step(cac_hip_setup(h, 1, 1));  // Verlet::setup: lists, then forces with energy and virial
+  return rc;
+}
+
+// the C ABI's code for an error a launcher raised (as the guarded entry points map them)
+int code_of(const HipError& e) {
+  if (e.code == hipErrorInvalidValue) {
+    if (e.where.find("overflow") != std::string::npos) return CAC_HIP_EOVERFLOW;
+    if (e.where.find("Non-numeric") != std::string::npos) return CAC_HIP_ENAN;
+    if (e.where.find("deformed") != std::string::npos) return CAC_HIP_EDEFORM;
+    return CAC_HIP_EARG;
+  }
+  return e.code == hipErrorOutOfMemory ? CAC_HIP_EOOM : CAC_HIP_EHIP;
+}
+
+void finish(cac_hip_server* s, Active& a, int status, const std::string& msg) {
+  Slot& sl = s->slots[a.slot];
+  Ctx& c = sl.h->c;
+  Outcome o;
+  o.r.status = status;
+  o.r.nsteps = a.req->r.nsteps;
+  o.r.atom_equivalents = (long)a.req->r.natoms + 8L * a.req->r.nelements;
+  o.r.t_submit = a.req->t_submit;
+  o.r.t_admit = a.t_admit;
+  o.r.t_done = s->now();
+  o.r.step_ms_mean = a.done > 0 ? 1000.0 * (o.r.t_done - a.t_first) / a.done : 0.0;
+  o.r.builds = c.ncalls;
+  if (status == 0) {
+    try {
+      pair_sums(c, 1, 1);
+      double ke = 0.0, t = 0.0;
+      thermo(c, &ke, &t);
+      o.r.pe = c.eng_vdwl;
+      o.r.ke = ke;
+      o.r.etotal = c.eng_vdwl + ke;
+      CAC_HIP(hipMemcpyAsync(c.h_flags, c.d_flags.get(), sizeof(ErrorFlags), hipMemcpyDeviceToHost, c.stream));
+      CAC_HIP(hipStreamSynchronize(c.stream));
+      if (c.h_flags->nan) o.r.status = CAC_HIP_ENAN;
+      if (a.req->r.keep_state) {
+        o.x.resize((size_t)3 * c.A);
+        o.nodex.resize((size_t)24 * c.E);
+        cac_hip_download_atoms(sl.h, c.A ? o.x.data() : nullptr, nullptr, nullptr);
+        if (c.E) cac_hip_download_elements(sl.h, o.nodex.data(), nullptr, nullptr, nullptr);
+      }
+    } catch (const HipError& e) {
+      o.r.status = CAC_HIP_EHIP;
+    }
+  }
+  o.done = true;
+  sl.busy = false;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (!msg.empty()) s->err = "replica " + std::to_string(a.req->id) + ": " + msg;
+  s->out[a.req->id] = std::move(o);
+  s->finished++;
+  s->cv_done.notify_all();
+}
+
+void serve(cac_hip_server* s) {
+  (void)hipSetDevice(s->device);
+  std::vector<Active> act;
+  std::vector<float> round_ms;
+  for (;;) {
+    // admission, in arrival order
+    for (;;) {
+      std::unique_ptr<Request> q;
+      int slot = -1;
+      {
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (act.empty())
+          s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+        if (s->queue.empty()) {
+          if (act.empty() && s->stop) return;
+          break;
+        }
+        double held = 0.0;
+        for (const Slot& sl : s->slots) held += sl.held;
+        const double need = s->queue.front()->need;
+        for (int k = 0; k < (int)s->slots.size(); k++) {
+          const Slot& sl = s->slots[k];
+          if (sl.busy) continue;
+          const double after = held - sl.held + std::max(sl.held, need);
+          if (after <= s->budget || act.empty()) {  // a lone replica is always tried
+            slot = k;
+            break;
+          }
+        }
+        if (slot < 0) break;
+        q = std::move(s->queue.front());
+        s->queue.pop_front();
+      }
+      Slot& sl = s->slots[slot];
+      sl.busy = true;
+      sl.held = std::max(sl.held, q->need);
+      double held = 0.0;
+      for (const Slot& x : s->slots) held += x.held;
+      std::string msg;
+      Active a;
+      a.slot = slot;
+      a.req = std::move(q);
+      a.t_admit = s->now();
+      const int rc = load(sl, *a.req, msg);
+      a.t_first = a.t_last = s->now();
+      if (rc) {
+        finish(s, a, rc, msg);
+        continue;
+      }
+      act.push_back(std::move(a));
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->peak_hbm = std::max(s->peak_hbm, held);
+      s->peak_active = std::max(s->peak_active, (int)act.size());
+    }
+    if (act.empty()) continue;
+    round_ms.clear();
+    // one round: queue the first half of a step everywhere, then resolve
+    for (Active& a : act) {
+      Ctx& c = s->slots[a.slot].h->c;
+      const int ev = a.done + 1 == a.req->r.nsteps ? 1 : 0;
+      try {
+        a.issued = step_issue(c, a.req->r.dt, ev);
+      } catch (const HipError& e) {
+        a.issued = false;
+        a.done = -1;  // failed
+        finish(s, a, code_of(e), e.where);
+        continue;
+      }
+      if (!a.issued) {
+        const double t = s->now();
+        round_ms.push_back((float)(1000.0 * (t - a.t_last)));
+        a.t_last = t;
+        a.done++;
+      }
+    }
+    for (Active& a : act) {
+      if (a.done < 0 || !a.issued) continue;
+      Ctx& c = s->slots[a.slot].h->c;
+      const int ev = a.done + 1 == a.req->r.nsteps ? 1 : 0;
+      try {
+        step_resolve(c, a.req->r.dt, ev);
+      } catch (const HipError& e) {
+        a.done = -1;
+        finish(s, a, code_of(e), e.where);
+        continue;
+      }
+      const double t = s->now();
+      round_ms.push_back((float)(1000.0 * (t - a.t_last)));
+      a.t_last = t;
+      a.done++;
+    }
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->interval_ms.insert(s->interval_ms.end(), round_ms.begin(), round_ms.end());
+    }
+    for (Active& a : act) {
+      if (a.done < 0) continue;
+      if (a.done >= a.req->r.nsteps) {
+        finish(s, a, 0, "");
+        a.done = -1;
+      }
+    }
+    act.erase(std::remove_if(act.begin(), act.end(), [](const Active& a) { return a.done < 0; }), act.end());
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cac_hip_server_create(cac_hip_server** out, int device, int slots, double hbm_budget_bytes) {
+  if (!out) return CAC_HIP_EARG;
+  *out = nullptr;
+  if (slots < 1 || slots > 256) return CAC_HIP_EARG;
+  auto* s = new cac_hip_server();
+  s->device = device;
+  s->t0 = Clock::now();
+  for (int k = 0; k < slots; k++) {
+    Slot sl;
+    const int rc = cac_hip_create(&sl.h, device, CAC_HIP_UNITS_METAL);
+    if (rc) {
+      for (Slot& x : s->slots) cac_hip_destroy(x.h);
+      delete s;
+      return rc;
+    }
+    s->slots.push_back(sl);
+  }
+  if (hbm_budget_bytes <= 0.0) {
+    size_t fr = 0, tot = 0;
+    (void)hipSetDevice(device);
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+    hbm_budget_bytes = 0.75 * (double)fr;
+  }
+  s->budget = hbm_budget_bytes;
+  s->th = std::thread(serve, s);
+  *out = s;
+  return CAC_HIP_OK;
+}
+
+int cac_hip_server_destroy(cac_hip_server* s) {
+  if (!s) return CAC_HIP_EARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  for (Slot& sl : s->slots) cac_hip_destroy(sl.h);
+  delete s;
+  return CAC_HIP_OK;
+}
+
+const char* cac_hip_server_last_error(const cac_hip_server* s) { return s ? s->err.c_str() : "null server"; }
+
+int cac_hip_server_pair_eam(cac_hip_server* s, int nrho, double drho, int nr, double dr, double cut,
+                            const double* frho, const double* rhor, const double* z2r) {
+  if (!s) return CAC_HIP_EARG;
+  if (s->submitted) return set_err(s, CAC_HIP_ESTATE, "set the pair style before the first submit");
+  for (Slot& sl : s->slots) {
+    const int rc = cac_hip_pair_eam(sl.h, nrho, drho, nr, dr, cut, frho, rhor, z2r);
+    if (rc) return set_err(s, rc, cac_hip_last_error(sl.h));
+  }
+  s->pair_set = true;
+  return CAC_HIP_OK;
+}
+
+int cac_hip_server_pair_lj(cac_hip_server* s, double eps, double sigma, double cut) {
+  if (!s) return CAC_HIP_EARG;
+  if (s->submitted) return set_err(s, CAC_HIP_ESTATE, "set the pair style before the first submit");
+  for (Slot& sl : s->slots) {
+    const int rc = cac_hip_pair_lj(sl.h, eps, sigma, cut);
+    if (rc) return set_err(s, rc, cac_hip_last_error(sl.h));
+  }
+  s->pair_set = true;
+  return CAC_HIP_OK;
+}
+
+int cac_hip_server_submit(cac_hip_server* s, const cac_hip_request* req, long* id) {
+  if (!s || !req) return CAC_HIP_EARG;
+  if (!s->pair_set) return set_err(s, CAC_HIP_ESTATE, "no pair style");
+  const cac_hip_request& r = *req;
+  if (r.ntypes < 1 || !r.mass || r.natoms < 0 || r.nelements < 0 || r.natoms + r.nelements == 0)
+    return set_err(s, CAC_HIP_EARG, "bad replica sizes");
+  if (r.nsteps < 1) return set_err(s, CAC_HIP_EARG, "a replica runs at least one step");
+  if (r.natoms > 0 && (!r.x || !r.type)) return set_err(s, CAC_HIP_EARG, "bad atom arrays");
+  if (r.nelements > 0 && (!r.ctype || !r.nodex)) return set_err(s, CAC_HIP_EARG, "bad element arrays");
+  auto q = std::make_unique<Request>();
+  q->r = r;
+  q->mass.assign(r.mass, r.mass + r.ntypes);
+  const size_t na = 3 * (size_t)r.natoms, ne = 24 * (size_t)r.nelements;
+  if (r.natoms) {
+    q->x.assign(r.x, r.x + na);
+    q->v.assign(na, 0.0);
+    if (r.v) q->v.assign(r.v, r.v + na);
+    q->type.assign(r.type, r.type + r.natoms);
+  }
+  if (r.nelements) {
+    q->nodex.assign(r.nodex, r.nodex + ne);
+    q->nodev.assign(ne, 0.0);
+    if (r.nodev) q->nodev.assign(r.nodev, r.nodev + ne);
+    q->ctype.assign(r.ctype, r.ctype + r.nelements);
+  }
+  q->etype.assign(r.nelements, 1);
+  q->r.mass = q->mass.data();
+  q->r.x = q->x.data();
+  q->r.v = q->v.data();
+  q->r.type = q->type.data();
+  q->r.nodex = q->nodex.data();
+  q->r.nodev = q->nodev.data();
+  q->r.ctype = q->ctype.data();
+  q->need = need_bytes(r);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    q->id = s->next_id++;
+    q->t_submit = s->now();
+    if (id) *id = q->id;
+    s->submitted = true;
+    s->queue.push_back(std::move(q));
+  }
+  s->cv_work.notify_all();
+  return CAC_HIP_OK;
+}
+
+int cac_hip_server_drain(cac_hip_server* s) {
+  if (!s) return CAC_HIP_EARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->finished == s->next_id; });
+  return CAC_HIP_OK;
+}
+
+int cac_hip_server_result(cac_hip_server* s, long id, cac_hip_result* res) {
+  if (!s || !res) return CAC_HIP_EARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->out.find(id);
+  if (it == s->out.end() || !it->second.done) {
+    s->err = "no finished replica with that id";
+    return CAC_HIP_ESTATE;
+  }
+  *res = it->second.r;
+  return CAC_HIP_OK;
+}
+
+int cac_hip_server_state(cac_hip_server* s, long id, double* x, double* nodex) {
+  if (!s) return CAC_HIP_EARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->out.find(id);
+  if (it == s->out.end() || !it->second.done) {
+    s->err = "no finished replica with that id";
+    return CAC_HIP_ESTATE;
+  }
+  const Outcome& o = it->second;
+  if (o.x.empty() && o.nodex.empty()) {
+    s->err = "replica was not submitted with keep_state";
+    return CAC_HIP_ESTATE;
+  }
+  if (x && !o.x.empty()) std::memcpy(x, o.x.data(), sizeof(double) * o.x.size());
+  if (nodex && !o.nodex.empty()) std::memcpy(nodex, o.nodex.data(), sizeof(double) * o.nodex.size());
+  return CAC_HIP_OK;
+}
+
+int cac_hip_server_stats(cac_hip_server* s, double* p50_ms, double* p99_ms, long* nsteps, double* atom_steps_per_s,
+                         int* peak_active, double* peak_hbm_bytes) {
+  if (!s) return CAC_HIP_EARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::vector<float> v = s->interval_ms;
+  auto pct = [&](double q) -> double {
+    if (v.empty()) return 0.0;
+    const size_t k = std::min(v.size() - 1, (size_t)(q * (double)(v.size() - 1) + 0.5));
+    std::nth_element(v.begin(), v.begin() + (long)k, v.end());
+    return v[k];
+  };
+  if (p50_ms) *p50_ms = pct(0.50);
+  if (p99_ms) *p99_ms = pct(0.99);
+  if (nsteps) *nsteps = (long)s->interval_ms.size();
+  // aggregate rate: atom-eq steps of the finished replicas over the union of
+  // their [admission, done] intervals
+  std::vector<std::pair<double, double>> span;
+  double work = 0.0;
+  for (const auto& kv : s->out) {
+    const cac_hip_result& r = kv.second.r;
+    if (!kv.second.done || r.status) continue;
+    span.push_back({r.t_admit, r.t_done});
+    work += (double)r.nsteps * (double)r.atom_equivalents;
+  }
+  std::sort(span.begin(), span.end());
+  double busy = 0.0, lo = -1.0, hi = -1.0;
+  for (const auto& iv : span) {
+    if (iv.first > hi) {
+      busy += hi - lo;
+      lo = iv.first;
+      hi = iv.second;
+    } else {
+      hi = std::max(hi, iv.second);
+    }
+  }
+  busy += hi - lo;
+  if (atom_steps_per_s) *atom_steps_per_s = busy > 0.0 ? work / busy : 0.0;
+  if (peak_active) *peak_active = s->peak_active;
+  if (peak_hbm_bytes) *peak_hbm_bytes = s->peak_hbm;
+  return CAC_HIP_OK;
+}
+
+}  // extern "C"

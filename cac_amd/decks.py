@@ -440,6 +440,67 @@ def cac_dislocation(box_cells=(20, 20, 26), core_cells: int = 1503, ncell: int =
                 name=f"cac_dislocation_box{'x'.join(map(str, box_cells))}_core{core_cells}_n{ncell}")
 
 
+def strain_uniaxial(deck: Deck, axis: int, eps: float) -> Deck:
+    """The deck stretched by (1 + eps) along one box axis, about boxlo: box,
+    atom positions and nodes scale together (a uniaxial pre-strain)."""
+    s = np.ones(3)
+    s[axis] += eps
+    lo = deck.boxlo
+    out = dataclasses.replace(deck, boxhi=lo + (deck.boxhi - lo) * s, x=lo + (deck.x - lo) * s)
+    if deck.nelements:
+        out.nodex = lo + (deck.nodex - lo) * s
+    return out
+
+
+def serving_box(size: float, ratio: float):
+    """Superlattice box (nx, ny, nz) and core cells of a CAC replica of about
+    ``size`` atom-equivalents whose atomistic core holds ``ratio`` times as many
+    cells as its coarse region (9^3-site cells: 729 atoms per core cell, 8 Gauss
+    points per element); boxes up to 3:1 are taken, a size error weighing
+    more than the aspect ratio."""
+    per_cell = (729.0 * ratio + 8.0) / (1.0 + ratio)
+    m = max(size / per_cell / 4.0, 1.0)
+    best = None
+    top = int(np.ceil(m ** (1.0 / 3.0))) + 1
+    for nx in range(1, top + 1):
+        for ny in range(nx, int(np.sqrt(m / nx)) + 2):
+            nz = max(ny, int(round(m / (nx * ny))))
+            cells = 4 * nx * ny * nz
+            core = int(round(ratio / (1.0 + ratio) * cells))
+            ae = 729 * core + 8 * (cells - core)
+            if nz > 3 * nx:
+                continue
+            key = abs(np.log(ae / size)) + 0.05 * np.log(nz / nx)
+            if best is None or key < best[0]:
+                best = (key, (nx, ny, nz), core)
+    return best[1], best[2]
+
+
+def serving_stream(n: int = 512, seed: int = 77, rate: float = 25.0, size_range=(2000.0, 200000.0),
+                   ratio_range=(0.05, 0.3), t_range=(100.0, 600.0), strain: float = 0.02):
+    """Replica requests of a serving run: Poisson arrivals at ``rate`` per
+    second; size log-uniform over ``size_range`` atom-equivalents; core/coarse
+    cell ratio, temperature and uniaxial pre-strain (random axis) uniform.
+    Returns a list of dicts (arrival s, size, ratio, T, axis, strain, seed);
+    :func:`serving_deck` builds the deck of one."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    out = []
+    for k in range(n):
+        out.append(dict(arrival=float(t[k]), size=float(np.exp(rng.uniform(*np.log(size_range)))),
+                        ratio=float(rng.uniform(*ratio_range)), T=float(rng.uniform(*t_range)),
+                        axis=int(rng.integers(0, 3)), strain=float(rng.uniform(-strain, strain)),
+                        seed=int(rng.integers(0, 2 ** 31))))
+    return out
+
+
+def serving_deck(spec: dict) -> Deck:
+    """The EAM CAC deck of one serving request (9^3-site Hex elements)."""
+    box, core = serving_box(spec["size"], spec["ratio"])
+    d = cac_eam(box_cells=box, core_cells=core, ncell=9, jitter=0.02, T=spec["T"], seed=spec["seed"])
+    return strain_uniaxial(d, spec["axis"], spec["strain"])
+
+
 def wrap_atoms(deck: Deck) -> Deck:
     L = deck.boxhi - deck.boxlo
     deck.x = deck.boxlo + np.mod(deck.x - deck.boxlo, L)

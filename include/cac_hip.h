@@ -141,6 +141,73 @@ int cac_hip_neigh_stats(cac_hip_ctx *ctx, int *ncalls, int *ndanger);
 int cac_hip_inner_stats(cac_hip_ctx *ctx, long *builds, double *mean_entries, double *delta, int *live_sites);
 int cac_hip_set_inner_skin(cac_hip_ctx *ctx, double delta);
 
+/* ---- serving: a resident loop over independent replicas (no counterpart in
+ * the reference, whose one process runs one model; each replica is a whole
+ * Verlet::run of its own model, verlet.cpp:218-360).
+ *
+ * The server owns `slots` device contexts, each with its own stream, and one
+ * service thread.  Submitted replicas wait in arrival order and are admitted
+ * when a slot is free and the HBM the replica needs (estimated from its
+ * atoms, elements and unit cells) fits the budget beside what the slots
+ * hold.  Admitted replicas are stepped round-robin: each round queues the
+ * first half of a step (kick, drift, displacement checks, site refresh) for
+ * every active replica, then resolves each one's check and queues the rest,
+ * so kernels of different replicas overlap on the device.  A slot keeps its
+ * buffers when its replica finishes (the next replica reuses them, no device
+ * frees while serving).  All replicas share one pair style, set once. */
+typedef struct cac_hip_server cac_hip_server;
+
+typedef struct {
+  int units;                 /* CAC_HIP_UNITS_METAL or CAC_HIP_UNITS_LJ */
+  double lo[3], hi[3];       /* orthogonal periodic box */
+  int ntypes;
+  const double *mass;        /* [ntypes] */
+  int ncell[3];              /* unit cells per element edge (Hex, apc 1, 2x2x2 Gauss cells) */
+  int natoms;
+  const double *x, *v;       /* [natoms][3] */
+  const int *type;           /* [natoms] */
+  int nelements;
+  const int *ctype;          /* [nelements] */
+  const double *nodex, *nodev; /* [nelements][8][3] */
+  double skin;
+  int precision;             /* CAC_HIP_FP64 or CAC_HIP_MIXED */
+  int nsteps;
+  double dt;
+  int keep_state;            /* 1: keep the final positions for cac_hip_server_state */
+} cac_hip_request;
+
+typedef struct {
+  int status;                /* 0, or the CAC_HIP_E* code that ended the replica */
+  int nsteps;
+  long atom_equivalents;     /* atoms + 8 x elements */
+  double t_submit, t_admit, t_done; /* seconds since the server started */
+  double step_ms_mean;       /* (t_done - first step) / nsteps */
+  double pe, ke, etotal;     /* thermo of the last step */
+  long builds;               /* neighbor builds during the run */
+} cac_hip_result;
+
+/* slots: concurrent replicas (1..256); hbm_budget_bytes <= 0: 3/4 of free HBM */
+int cac_hip_server_create(cac_hip_server **srv, int device, int slots, double hbm_budget_bytes);
+int cac_hip_server_destroy(cac_hip_server *srv);
+/* the last error: of a server call, or the message of the last replica that failed */
+const char *cac_hip_server_last_error(const cac_hip_server *srv);
+/* the pair style of every replica (as cac_hip_pair_eam / cac_hip_pair_lj) */
+int cac_hip_server_pair_eam(cac_hip_server *srv, int nrho, double drho, int nr, double dr, double cut,
+                            const double *frho, const double *rhor, const double *z2r);
+int cac_hip_server_pair_lj(cac_hip_server *srv, double eps, double sigma, double cut);
+/* queue a replica; the arrays are copied before the call returns */
+int cac_hip_server_submit(cac_hip_server *srv, const cac_hip_request *req, long *id);
+/* block until every submitted replica has finished */
+int cac_hip_server_drain(cac_hip_server *srv);
+int cac_hip_server_result(cac_hip_server *srv, long id, cac_hip_result *out);
+/* final positions of a replica submitted with keep_state ([natoms][3], [nelements][8][3]) */
+int cac_hip_server_state(cac_hip_server *srv, long id, double *x, double *nodex);
+/* step intervals of all replica steps so far (ms), as seen by the service
+ * thread: p50, p99, count; aggregate atom-eq steps per second over the time
+ * any replica was active; peak slots in use; peak HBM the slots held */
+int cac_hip_server_stats(cac_hip_server *srv, double *p50_ms, double *p99_ms, long *nsteps, double *atom_steps_per_s,
+                         int *peak_active, double *peak_hbm_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,92 @@
+"""The serving loop (cac_hip_server): replica requests on the CPU; on the GPU,
+each replica's result is the same alone and in the mix, whatever the slot it
+lands in, the replicas it shares the device with and the admission order."""
+
+import numpy as np
+import pytest
+
+from cac_amd import decks
+
+TAB = decks.cu_eam_tables()
+
+
+def test_serving_stream_follows_its_distributions():
+    sp = decks.serving_stream(n=512, seed=77, rate=25.0)
+    t = np.array([s["arrival"] for s in sp])
+    assert np.all(np.diff(t) > 0)
+    gaps = np.diff(np.concatenate([[0.0], t]))
+    assert abs(gaps.mean() - 1 / 25.0) < 0.2 / 25.0  # Poisson arrivals at 25 per second
+    size = np.array([s["size"] for s in sp])
+    assert size.min() >= 2000 and size.max() <= 200000
+    ls = np.log(size)
+    assert abs(ls.mean() - 0.5 * (np.log(2000) + np.log(200000))) < 0.25  # log-uniform
+    for key, lo, hi in (("ratio", 0.05, 0.3), ("T", 100, 600), ("strain", -0.02, 0.02)):
+        v = np.array([s[key] for s in sp])
+        assert v.min() >= lo and v.max() <= hi
+
+
+def test_serving_deck_matches_its_request():
+    spec = dict(arrival=0.0, size=20000.0, ratio=0.2, T=400.0, axis=1, strain=0.015, seed=5)
+    d = decks.serving_deck(spec)
+    assert abs(d.atom_equivalents / 20000.0 - 1) < 0.25
+    box, core = decks.serving_box(20000.0, 0.2)
+    L0 = 9 * 3.615 * np.array(box, dtype=float)
+    assert np.allclose(d.boxhi - d.boxlo, L0 * np.array([1, 1.015, 1]))
+    assert d.natoms == 729 * core
+
+
+def _specs(n, seed):
+    rng = np.random.default_rng(seed)
+    return [dict(arrival=0.0, size=float(rng.uniform(2000, 9000)), ratio=float(rng.uniform(0.05, 0.3)),
+                 T=float(rng.uniform(100, 600)), axis=int(rng.integers(0, 3)),
+                 strain=float(rng.uniform(-0.02, 0.02)), seed=int(rng.integers(0, 2 ** 31))) for _ in range(n)]
+
+
+@pytest.mark.gpu
+def test_replicas_same_alone_and_in_the_mix():
+    """Six replicas of uneven size, temperature and strain, 60 mixed steps each:
+    run alone through the resident loop, then together on three slots (so
+    slots are reused by replicas of other sizes), then one at a time under a
+    budget that admits a single replica: final positions bitwise equal, final
+    energies equal."""
+    import cac_amd
+    specs = _specs(6, 11)
+    dks = [decks.serving_deck(s) for s in specs]
+    alone = []
+    for d in dks:
+        m = cac_amd.Model(d, "eam", skin=2.0, eam_tables=TAB, precision=cac_amd.MIXED)
+        m.setup()
+        th = m.run(60, 0.001)
+        alone.append((m.atoms()[0], m.elements()["nodex"] if d.nelements else None, th[-1]))
+        m.close()
+    for slots, budget in ((3, 0.0), (2, 1.0)):
+        srv = cac_amd.Server(slots=slots, hbm_budget_bytes=budget, eam_tables=TAB)
+        ids = [srv.submit(d, 60, 0.001, skin=2.0, keep_state=True) for d in dks]
+        srv.drain()
+        st = srv.stats()
+        assert st["steps"] == 6 * 60
+        assert st["peak_active"] == (3 if budget == 0.0 else 1), st
+        for rid, d, (x0, nx0, th0) in zip(ids, dks, alone):
+            r = srv.result(rid)
+            assert r["status"] == 0 and r["nsteps"] == 60
+            x, nx = srv.state(rid, d)
+            assert np.array_equal(x, x0)
+            if d.nelements:
+                assert np.array_equal(nx, nx0)
+            assert r["pe"] == th0[1] and r["ke"] == th0[2], (r, th0)
+        srv.close()
+
+
+@pytest.mark.gpu
+def test_server_rejects_bad_requests_and_reports_failures():
+    import cac_amd
+    srv = cac_amd.Server(slots=2, eam_tables=TAB)
+    d = decks.serving_deck(_specs(1, 3)[0])
+    with pytest.raises(cac_amd.CacHipError):
+        srv.submit(d, 0, 0.001)
+    rid = srv.submit(d, 5, 0.001)
+    srv.drain()
+    assert srv.result(rid)["status"] == 0
+    with pytest.raises(cac_amd.CacHipError):
+        srv.state(rid, d)  # not kept
+    srv.close()
