@@ -292,6 +292,7 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     if (const char* e = std::getenv("CAC_HIP_TPR_FORCE")) c.tpr_force = std::atoi(e);
     if (const char* e = std::getenv("CAC_HIP_GRID")) c.grid_cap = std::max(8, std::atoi(e));
     if (const char* e = std::getenv("CAC_HIP_PAIR_SHAPE")) c.pair_shape = std::atoi(e);
+    if (const char* e = std::getenv("CAC_HIP_DENS_NT")) c.dens_nt = std::atoi(e) == 1024 ? 1024 : 512;
     CAC_HIP(hipDeviceGetAttribute(&c.ncu, hipDeviceAttributeMultiprocessorCount, device));
     if (units == CAC_HIP_UNITS_LJ) {
       c.boltz = 1.0;

@@ -262,6 +262,7 @@ struct Ctx {
   int tpr_force = 4;     // lanes per neighbor row in the force passes (EAM, LJ)
   int grid_cap = 4096;   // max workgroups of the persistent pair kernels
   int pair_shape = 1;    // EAM pass shape: 0/1/2/3 = (512|1024 threads) x (4|8 entries in flight)
+  int dens_nt = 512;     // workgroup size of the EAM density pass (512 or 1024)
   int ncu = 256;         // compute units of the device
   std::map<std::pair<const void*, size_t>, int> resident;  // workgroups per CU of a kernel at an LDS size
   int nodef_style = 0;   // 0 consistent, 1 lumped, 2 node/only

@@ -696,97 +696,100 @@ constexpr int MIX_SHIFT = 25;
 // site, with the cutoff widened by 1e-5 relative: that covers FP32 rounding
 // of the pair vector (below 1e-6 relative at these lengths), so the inner
 // list still holds every pair within rc + delta.  The FP64 path tests the FP64
-// records.
+// records.  A mixed record carries its site's packed cell in w, so one gather
+// per entry gives both the position and the cell.
+//
+// 16 lanes per row, PU entries per lane per batch (PU = 9: one batch holds the
+// 134-entry rows of an FCC crystal at rc + skin).  List words are read at an
+// index clamped into the row and masked past its end: no branch.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site, const float4* __restrict__ site32f,
-                                               const int* __restrict__ scell, const int* __restrict__ row_site,
-                                               const int* __restrict__ row_point, const int* __restrict__ nnum,
-                                               const unsigned* __restrict__ nlist, int R, int cap, Grid g,
-                                               double cutsq_in, int* __restrict__ innum,
+                                               const int* __restrict__ row_site, const int* __restrict__ row_point,
+                                               const int* __restrict__ nnum, const unsigned* __restrict__ nlist,
+                                               int R, int cap, Grid g, double cutsq_in, int* __restrict__ innum,
                                                unsigned* __restrict__ ilist, const double* __restrict__ x,
                                                double* __restrict__ xhold_in, int* __restrict__ live) {
-  constexpr int PU = 4;  // 16-entry groups in flight per lane group
+  constexpr int PU = MIXED ? 9 : 6;
   const int lane = threadIdx.x & 15;
   const int gbase = (threadIdx.x & 63) & ~15;
   // grid-stride over rows, so a step that does not re-prune launches few workgroups
   const long rstride = ((long)gridDim.x * blockDim.x) >> 4;
   const long rfirst = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 4;
-  for (long rl = rfirst; rl < R; rl += rstride) {
-  const int r = (int)rl;
-  if (lane < 3) {
-    const size_t i = (size_t)lane * R + row_point[r];
-    xhold_in[i] = x[i];
-  }
-  if (lane == 3) live[row_site[r]] = 1;
-  const int n = nnum[r];
-  const int k = row_site[r];
-  const double4 pk = MIXED ? make_double4(0.0, 0.0, 0.0, 0.0) : site[k];
-  const float4 pk32 = MIXED ? site32f[k] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float cut32 = (float)(cutsq_in * (1.0 + 1e-5));
   const float cs0 = (float)(g.L[0] / g.nc[0]), cs1 = (float)(g.L[1] / g.nc[1]), cs2 = (float)(g.L[2] / g.nc[2]);
-  const int cp = scell[k];
-  const int icx = cell_ix(cp), icy = cell_iy(cp), icz = cell_iz(cp);
-  const unsigned* row = nlist + (size_t)r * cap;
-  unsigned* out = ilist + (size_t)r * cap;
-  int count = 0;
-  for (int base = 0; base < n; base += 16 * PU) {
-    unsigned ent[PU];
-    double4 sj[PU];
-    float4 sj32[PU];
-    int cj[PU];
-#pragma unroll
-    for (int u = 0; u < PU; u++) {
-      const int jj = base + 16 * u + lane;
-      ent[u] = jj < n ? row[jj] : (unsigned)k | (13u << IMG_SHIFT);
+  for (long rl = rfirst; rl < R; rl += rstride) {
+    const int r = (int)rl;
+    const int k = row_site[r];
+    if (lane < 3) {
+      const size_t i = (size_t)lane * R + row_point[r];
+      xhold_in[i] = x[i];
     }
+    if (lane == 3) live[k] = 1;
+    const int n = nnum[r];
+    const double4 pk = MIXED ? make_double4(0.0, 0.0, 0.0, 0.0) : site[k];
+    const float4 pk32 = MIXED ? site32f[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int cp = MIXED ? __float_as_int(pk32.w) : 0;
+    const int icx = cell_ix(cp), icy = cell_iy(cp), icz = cell_iz(cp);
+    const unsigned* row = nlist + (size_t)r * cap;
+    unsigned* out = ilist + (size_t)r * cap;
+    int count = 0;
+    for (int base = 0; base < n; base += 16 * PU) {
+      unsigned ent[PU];
 #pragma unroll
-    for (int u = 0; u < PU; u++) {
-      if (MIXED) {
-        sj32[u] = site32f[ent[u] & SITE_MASK];
-        cj[u] = scell[ent[u] & SITE_MASK];
-      } else {
-        sj[u] = site[ent[u] & SITE_MASK];
+      for (int u = 0; u < PU; u++) {
+        const int jj = base + 16 * u + lane;
+        ent[u] = row[jj < n ? jj : n - 1];
+      }
+      double4 sj[MIXED ? 1 : PU];
+      float4 sj32[MIXED ? PU : 1];
+#pragma unroll
+      for (int u = 0; u < PU; u++) {
+        if (MIXED)
+          sj32[u] = site32f[ent[u] & SITE_MASK];
+        else
+          sj[u] = site[ent[u] & SITE_MASK];
+      }
+#pragma unroll
+      for (int u = 0; u < PU; u++) {
+        const int jj = base + 16 * u + lane;
+        const unsigned t = ent[u] >> IMG_SHIFT;
+        const int tx = (int)(t % 3u) - 1, ty = (int)((t / 3u) % 3u) - 1, tz = (int)(t / 9u) - 1;
+        bool keep;
+        unsigned eout = ent[u];
+        if (MIXED) {
+          const int cj = __float_as_int(sj32[u].w);
+          const int ox = cell_ix(cj) + tx * g.nc[0] - icx;
+          const int oy = cell_iy(cj) + ty * g.nc[1] - icy;
+          const int oz = cell_iz(cj) + tz * g.nc[2] - icz;
+          const float dx = (pk32.x - sj32[u].x) - (float)ox * cs0;
+          const float dy = (pk32.y - sj32[u].y) - (float)oy * cs1;
+          const float dz = (pk32.z - sj32[u].z) - (float)oz * cs2;
+          keep = jj < n && dx * dx + dy * dy + dz * dz <= cut32;
+          const unsigned dcode = (unsigned)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
+          eout = (ent[u] & SITE_MASK) | (dcode << MIX_SHIFT);
+        } else {
+          const double dx = pk.x - (sj[u].x + (double)tx * g.L[0]);
+          const double dy = pk.y - (sj[u].y + (double)ty * g.L[1]);
+          const double dz = pk.z - (sj[u].z + (double)tz * g.L[2]);
+          keep = jj < n && dx * dx + dy * dy + dz * dz <= cutsq_in;
+        }
+        const unsigned long long m = __ballot(keep);
+        const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
+        const int rank = __popc(gm & ((1u << lane) - 1u));
+        if (keep) {
+          out[count + rank] = eout;
+          live[ent[u] & SITE_MASK] = 1;
+        }
+        count += __popc(gm);
       }
     }
-#pragma unroll
-    for (int u = 0; u < PU; u++) {
-      const int jj = base + 16 * u + lane;
-      const unsigned t = ent[u] >> IMG_SHIFT;
-      const int tx = (int)(t % 3u) - 1, ty = (int)((t / 3u) % 3u) - 1, tz = (int)(t / 9u) - 1;
-      bool keep;
-      unsigned eout = ent[u];
-      if (MIXED) {
-        const int ox = cell_ix(cj[u]) + tx * g.nc[0] - icx;
-        const int oy = cell_iy(cj[u]) + ty * g.nc[1] - icy;
-        const int oz = cell_iz(cj[u]) + tz * g.nc[2] - icz;
-        const float dx = (pk32.x - sj32[u].x) - (float)ox * cs0;
-        const float dy = (pk32.y - sj32[u].y) - (float)oy * cs1;
-        const float dz = (pk32.z - sj32[u].z) - (float)oz * cs2;
-        keep = jj < n && dx * dx + dy * dy + dz * dz <= cut32;
-        const unsigned dcode = (unsigned)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
-        eout = (ent[u] & SITE_MASK) | (dcode << MIX_SHIFT);
-      } else {
-        const double dx = pk.x - (sj[u].x + (double)tx * g.L[0]);
-        const double dy = pk.y - (sj[u].y + (double)ty * g.L[1]);
-        const double dz = pk.z - (sj[u].z + (double)tz * g.L[2]);
-        keep = jj < n && dx * dx + dy * dy + dz * dz <= cutsq_in;
-      }
-      const unsigned long long m = __ballot(keep);
-      const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
-      const int rank = __popc(gm & ((1u << lane) - 1u));
-      if (keep) {
-        out[count + rank] = eout;
-        live[ent[u] & SITE_MASK] = 1;
-      }
-      count += __popc(gm);
-    }
-  }
-  if (lane == 0) innum[r] = count;
+    if (lane == 0) innum[r] = count;
   }
 }
 
 // live-site count, dense index of every row's site, and the inner-list
-// entries re-pointed at the dense records (16 lanes per row)
+// entries re-pointed at the dense records (16 lanes per row; a lane reads all
+// its entries, then all their dense indices, then writes)
 __global__ void k_live_count(const int* __restrict__ flag, const int* __restrict__ idx, int S, int* __restrict__ n) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *n = S ? idx[S - 1] + flag[S - 1] : 0;
 }
@@ -794,6 +797,7 @@ __global__ void k_live_count(const int* __restrict__ flag, const int* __restrict
 __global__ __launch_bounds__(256) void k_live_remap(const int* __restrict__ idx, const int* __restrict__ row_site,
                                                     int* __restrict__ row_live, const int* __restrict__ innum,
                                                     unsigned* __restrict__ ilist, int R, int cap, unsigned mask) {
+  constexpr int RU = 6;
   const int lane = threadIdx.x & 15;
   const long rstride = ((long)gridDim.x * blockDim.x) >> 4;
   const long rfirst = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 4;
@@ -802,9 +806,21 @@ __global__ __launch_bounds__(256) void k_live_remap(const int* __restrict__ idx,
     if (lane == 0) row_live[r] = idx[row_site[r]];
     unsigned* row = ilist + (size_t)r * cap;
     const int n = innum[r];
-    for (int i = lane; i < n; i += 16) {
-      const unsigned e = row[i];
-      row[i] = (unsigned)idx[e & mask] | (e & ~mask);
+    for (int b = 0; b < n; b += 16 * RU) {
+      unsigned e[RU];
+      int d[RU];
+#pragma unroll
+      for (int u = 0; u < RU; u++) {
+        const int i = b + 16 * u + lane;
+        e[u] = row[i < n ? i : n - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < RU; u++) d[u] = idx[e[u] & mask];
+#pragma unroll
+      for (int u = 0; u < RU; u++) {
+        const int i = b + 16 * u + lane;
+        if (i < n) row[i] = (unsigned)d[u] | (e[u] & ~mask);
+      }
     }
   }
 }
@@ -817,6 +833,9 @@ void build_inner(Ctx& c) {
   c.inner_cutsq = rin * rin;
   Grid g = grid_of(c);
   c.icap = c.cap;
+  // the pair passes address the inner list with 32-bit byte offsets
+  if ((size_t)c.P * (size_t)c.icap * 4 >= (1ull << 32))
+    throw HipError{hipErrorInvalidValue, "inner neighbor list larger than 4 GiB (too many points for one GPU)"};
   c.innum.reserve(c.P);
   c.ilist.reserve((size_t)c.icap * c.P);
   c.live_flag.reserve(c.S);
@@ -828,13 +847,13 @@ void build_inner(Ctx& c) {
   const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
   unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());
   if (mixed)
-    k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.scell.get(), c.row_site.get(),
-                                                 c.row_point.get(), c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq,
-                                                 c.innum.get(), il, c.x.get(), c.xhold_in.get(), c.live_flag.get());
+    k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.row_site.get(), c.row_point.get(),
+                                                 c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
+                                                 c.x.get(), c.xhold_in.get(), c.live_flag.get());
   else
-    k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.scell.get(), c.row_site.get(),
-                                                  c.row_point.get(), c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq,
-                                                  c.innum.get(), il, c.x.get(), c.xhold_in.get(), c.live_flag.get());
+    k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.row_site.get(), c.row_point.get(),
+                                                  c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
+                                                  c.x.get(), c.xhold_in.get(), c.live_flag.get());
   scan_exclusive(c, c.live_flag.get(), c.live_idx.get(), c.S, nullptr);
   k_live_count<<<1, 64, 0, c.stream>>>(c.live_flag.get(), c.live_idx.get(), c.S, c.d_nlive.get());
   compact_live(c);

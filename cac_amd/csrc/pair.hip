@@ -112,6 +112,9 @@ __device__ __forceinline__ double row_sum(double v) {
 // cell-offset table of the mixed path in LDS: code (dx+2) + 5(dy+2) + 25(dz+2)
 // -> (dx, dy, dz) * cell edge, so the pair vector is (u_i - u_j) - offset
 constexpr int NOFF = 125;
+// the mixed passes keep this table at the start of LDS (a lookup is then a
+// shift and a load) and their spline table NOFF_PAD float4 after it
+constexpr int NOFF_PAD = 128;
 
 __device__ __forceinline__ void stage_offsets(float4* offs, const float* cs) {
   for (int t = threadIdx.x; t < NOFF; t += blockDim.x) {
@@ -197,9 +200,12 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, U
     for (int c = 0; c < 3; c++) y[c] = y[c] + sh[c];
     const int k = kk[t];
     if (MIXED) {
+      // the full records carry the site's packed cell in w (the prune reads
+      // it there); the dense ones get F'(rho) in w from the embedding pass
       const float4 r32 = make_float4((float)(y[0] - (fr.lo[0] + cell_ix(cc[t]) * fr.cs[0])),
                                      (float)(y[1] - (fr.lo[1] + cell_iy(cc[t]) * fr.cs[1])),
-                                     (float)(y[2] - (fr.lo[2] + cell_iz(cc[t]) * fr.cs[2])), 0.0f);
+                                     (float)(y[2] - (fr.lo[2] + cell_iz(cc[t]) * fr.cs[2])),
+                                     full ? __int_as_float(cc[t]) : 0.0f);
       (full ? site32f : site32)[k] = r32;
     } else {
       (full ? site : site_live)[k] = make_double4(y[0], y[1], y[2], 0.0);
@@ -223,7 +229,8 @@ __global__ __launch_bounds__(256) void k_live_compact(const int* __restrict__ fl
   lsimg[i] = simg[k];
   lscell[i] = scell[k];
   if (MIXED) {
-    site32[i] = site32f[k];
+    const float4 r = site32f[k];
+    site32[i] = make_float4(r.x, r.y, r.z, 0.0f);
   } else {
     const double4 y = site[k];
     site_live[i] = make_double4(y.x, y.y, y.z, 0.0);
@@ -358,16 +365,34 @@ __device__ __forceinline__ void stage_table_strided(V* lds, const V* __restrict_
 
 constexpr int UNROLL = 4;
 
+// U entries per lane per batch: lane l of the row takes entries l, l + TPR,
+// ...  List words are read at an index clamped into the row (n >= 1 here):
+// past the row's end a lane re-reads its last entry, a real site, which the
+// body masks out.  The load is always in bounds and needs no branch or select.
+// The list is addressed by 32-bit byte offsets from its base (a scalar
+// pointer), so a word load is one shift and a load: the host keeps every list
+// below 4 GiB.
+__device__ __forceinline__ unsigned list_word(const unsigned* __restrict__ base, unsigned byte_off) {
+  return *reinterpret_cast<const unsigned*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+template <int TPR, int U>
+__device__ __forceinline__ void row_words(const unsigned* __restrict__ base, unsigned row0, int n, int j0,
+                                          unsigned* e) {
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int jj = j0 + u * TPR;
+    e[u] = list_word(base, (row0 + (unsigned)(jj < n ? jj : n - 1)) << 2);
+  }
+}
+
+// row r of the list starts at word row0 = r * cap
 template <int TPR, int U, typename Site, typename Fetch, typename Body>
-__device__ __forceinline__ void walk_row(const unsigned* __restrict__ row, int n, int lane, unsigned self,
+__device__ __forceinline__ void walk_row(const unsigned* __restrict__ base, unsigned row0, int n, int lane,
                                          Fetch fetch, Body body) {
   for (int j0 = lane; j0 < n; j0 += U * TPR) {
     unsigned e[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int jj = j0 + u * TPR;
-      e[u] = jj < n ? row[jj] : self;
-    }
+    row_words<TPR, U>(base, row0, n, j0, e);
     Site sj[U];
 #pragma unroll
     for (int u = 0; u < U; u++) sj[u] = fetch(e[u]);
@@ -383,13 +408,13 @@ template <int TPR, bool MIXED, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __restrict__ tab, double rdr, int nr,
                                                     double* __restrict__ rho_out, Gate g) {
   if (g.skip()) return;
+  __shared__ float4 offs[NOFF_PAD];  // static: its address is a constant of the code
   extern __shared__ float4 lds_raw[];
-  float4* lds32 = lds_raw;
+  float4* tab32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
-  const float4* offs = lds32 + (nr + 1);
   if (MIXED) {
-    stage_offsets(lds32 + (nr + 1), a.cs);
-    stage_table(lds32, static_cast<const float4*>(tab), nr + 1);
+    stage_offsets(offs, a.cs);
+    stage_table(tab32, static_cast<const float4*>(tab), nr + 1);
   } else {
     stage_table_strided(lds64, static_cast<const double4*>(tab), nr + 1);
   }
@@ -404,27 +429,29 @@ __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __re
     if (r < a.R) {
       const int k = a.row_site[r];
       const int n = a.nnum[r];
-      const unsigned* row = a.nlist + (size_t)r * a.cap;
+      const unsigned row0 = (unsigned)r * (unsigned)a.cap;
       if (MIXED) {
         const float4 pi = a.site32[k];
         walk_row<TPR, U, float4>(
-            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
+            a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
             [&](unsigned ent, const float4& sj, bool valid) {
               const float4 o = offs[ent >> MIX_SHIFT];
               const float dx = (pi.x - sj.x) - o.x;
               const float dy = (pi.y - sj.y) - o.y;
               const float dz = (pi.z - sj.z) - o.z;
               const float rsq = dx * dx + dy * dy + dz * dz;
+              // entries past the row's end or beyond rc count 0 by a mask
+              // multiply: a select would become a branch per entry
               const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
               float p;
               const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, nr, p);
-              const float4 c = lds32[m];
+              const float4 c = tab32[m];
               acc += (double)(okf * (((c.x * p + c.y) * p + c.z) * p + c.w));
             });
       } else {
         const double4 pi = a.site[k];
         walk_row<TPR, U, double4>(
-            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
+            a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
             [&](unsigned ent, const double4& sj, bool valid) {
               double shx, shy, shz;
               img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
@@ -514,13 +541,13 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
                                                   double* __restrict__ gf, int P, int E8,
                                                   double* __restrict__ erow, Gate g) {
   if (g.skip()) return;
+  __shared__ float4 offs[NOFF_PAD];  // static: its address is a constant of the code
   extern __shared__ float4 lds_raw[];
-  float4* lds32 = lds_raw;
+  float4* tab32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
-  const float4* offs = lds32 + 2 * (q.nr + 1);
   if (MIXED) {
-    stage_offsets(lds32 + 2 * (q.nr + 1), a.cs);
-    stage_table(lds32, static_cast<const float4*>(q.tab), 2 * (q.nr + 1));
+    stage_offsets(offs, a.cs);
+    stage_table(tab32, static_cast<const float4*>(q.tab), 2 * (q.nr + 1));
   } else {
     stage_table(lds64, static_cast<const double4*>(q.tab), 2 * (q.nr + 1));
   }
@@ -528,7 +555,6 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
   const int rsub = threadIdx.x / TPR;
   const Sched s = sched(a.nchunks);
   const float rdr32 = (float)q.rdr;
-  const float k3 = 3.0f * rdr32, k2 = 2.0f * rdr32;
   for (long ch = s.first; ch < s.end; ch += s.stride) {
     const int r = (int)(ch * (NT / TPR)) + rsub;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -537,12 +563,12 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
     if (r < a.R) {
       k = a.row_site[r];
       const int n = a.nnum[r];
-      const unsigned* row = a.nlist + (size_t)r * a.cap;
+      const unsigned row0 = (unsigned)r * (unsigned)a.cap;
       if (MIXED) {
         const float4 pi = a.site32[k];
         const float cut32 = (float)a.cutsq;
         walk_row<TPR, U, float4>(
-            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
+            a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
             [&](unsigned ent, const float4& sj, bool valid) {
               const float4 o = offs[ent >> MIX_SHIFT];
               const float dx = (pi.x - sj.x) - o.x;
@@ -556,17 +582,22 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
               const float rr = rsq * recip;
               float p;
               const int m = knot_r32(rr, rdr32, q.nr, p);
-              const float4 cz = lds32[2 * m];      // z2r value c3..c6
-              const float4 cd = lds32[2 * m + 1];  // rhor derivative c0..c2, z2r c2
+              const float4 cz = tab32[2 * m];      // z2r value c3..c6
+              const float4 cd = tab32[2 * m + 1];  // rhor derivative c0..c2, z2r c2
               const float rhop = (cd.x * p + cd.y) * p + cd.z;
-              const float z2p = ((cz.x * k3) * p + cz.y * k2) * p + cd.w;
-              const float z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
+              // z2 = ((c3 p + c4) p + c5) p + c6 by Horner, and dz2/dp from
+              // the same partial sums; z2' = dz2/dp * rdr
+              const float h1 = cz.x * p + cz.y;
+              const float h2 = h1 * p + cz.z;
+              const float z2 = h2 * p + cz.w;
+              const float dz2 = (cz.x * p + h1) * p + h2;
               const float phi = z2 * recip;
-              const float phip = (z2p - phi) * recip;
+              const float phip = (dz2 * rdr32 - phi) * recip;
               const float fpair = okf * (-((pi.w + sj.w) * rhop + phip) * recip);
-              fx += (double)(dx * fpair);
-              fy += (double)(dy * fpair);
-              fz += (double)(dz * fpair);
+              const double fp = (double)fpair;
+              fx = __builtin_fma((double)dx, fp, fx);
+              fy = __builtin_fma((double)dy, fp, fy);
+              fz = __builtin_fma((double)dz, fp, fz);
               if (EV) {
                 ephi += (double)(okf * phi);
                 v0 += (double)(dx * dx * fpair);
@@ -580,7 +611,7 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
       } else {
         const double4 pi = a.site[k];
         walk_row<TPR, U, double4>(
-            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
+            a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
             [&](unsigned ent, const double4& sj, bool valid) {
               double shx, shy, shz;
               img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
@@ -687,12 +718,12 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
     if (r < a.R) {
       const int k = a.row_site[r];
       const int n = a.nnum[r];
-      const unsigned* row = a.nlist + (size_t)r * a.cap;
+      const unsigned row0 = (unsigned)r * (unsigned)a.cap;
       if (MIXED) {
         const float4 pi = a.site32[k];
         const float cut32 = (float)a.cutsq;
         walk_row<TPR, UNROLL, float4>(
-            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
+            a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
             [&](unsigned ent, const float4& sj, bool valid) {
               float dx, dy, dz;
               mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
@@ -702,9 +733,10 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
               const float r6inv = r2inv * r2inv * r2inv;
               const float forcelj = r6inv * (l1 * r6inv - l2);
               const float fpair = okf * (forcelj * r2inv);
-              fx += (double)(dx * fpair);
-              fy += (double)(dy * fpair);
-              fz += (double)(dz * fpair);
+              const double fp = (double)fpair;
+              fx = __builtin_fma((double)dx, fp, fx);
+              fy = __builtin_fma((double)dy, fp, fy);
+              fz = __builtin_fma((double)dz, fp, fz);
               if (EV) {
                 ev += (double)(okf * (r6inv * (l3 * r6inv - l4)));
                 v0 += (double)(dx * dx * fpair);
@@ -718,7 +750,7 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
       } else {
         const double4 pi = a.site[k];
         walk_row<TPR, UNROLL, double4>(
-            row, n, lane, (unsigned)k, [&](unsigned e) { return a.site[e & SITE_MASK]; },
+            a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
             [&](unsigned ent, const double4& sj, bool valid) {
               double shx, shy, shz;
               img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
@@ -839,7 +871,7 @@ static int resident_grid(Ctx& c, K kernel, int nt, size_t lds, long nchunks) {
 template <int TPR, bool MIXED, int NT, int U>
 static void run_density(Ctx& c, Gate g) {
   const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
-  const size_t lds_rho = knot * (c.nr + 1) + (MIXED ? sizeof(float4) * NOFF : 0);
+  const size_t lds_rho = knot * (c.nr + 1);
   const PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
   const void* tab_rho = MIXED ? (const void*)c.tab32r.get() : (const void*)c.tab64.get();
   ScopedTimer tm(c, "eam_density");
@@ -855,7 +887,7 @@ static void run_force(Ctx& c, int eflag, int vflag, Gate g) {
   double* gf = c.gf.get();
   double* er = ev ? c.erow.get() : nullptr;
   const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
-  const size_t lds_force = 2 * knot * (c.nr + 1) + (MIXED ? sizeof(float4) * NOFF : 0);
+  const size_t lds_force = 2 * knot * (c.nr + 1);
   const PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
   EamForceArgs q;
   q.tab = MIXED ? (const void*)c.tab32f.get() : (const void*)c.tab64.get();
@@ -882,13 +914,24 @@ static void run_force(Ctx& c, int eflag, int vflag, Gate g) {
 
 // the three EAM passes; lanes per row chosen separately for the density and
 // the force pass
+template <bool MIXED, int NTD, int U>
+static void run_density_tpr(Ctx& c, Gate g) {
+  switch (c.tpr) {
+    case 4: run_density<4, MIXED, NTD, U>(c, g); break;
+    case 16: run_density<16, MIXED, NTD, U>(c, g); break;
+    default: run_density<8, MIXED, NTD, U>(c, g); break;
+  }
+}
+
+// The density pass stages half the table of the force pass, so more of its
+// workgroups fit on a CU: its workgroup size (dens_nt) is chosen apart from
+// the force pass's (NT).
 template <bool MIXED, int NT, int U>
 static void run_eam(Ctx& c, int eflag, int vflag, Gate g) {
-  switch (c.tpr) {
-    case 4: run_density<4, MIXED, NT, U>(c, g); break;
-    case 16: run_density<16, MIXED, NT, U>(c, g); break;
-    default: run_density<8, MIXED, NT, U>(c, g); break;
-  }
+  if (c.dens_nt == 1024)
+    run_density_tpr<MIXED, 1024, U>(c, g);
+  else
+    run_density_tpr<MIXED, 512, U>(c, g);
   {
     ScopedTimer tm(c, "eam_embed");
     k_embed<MIXED><<<grid_for(c.S, 256 * RSPT), 256, 0, c.stream>>>(c.rho.get(), ucell_grid(c), c.lsdesc.get(),
