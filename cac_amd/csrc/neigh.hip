@@ -710,13 +710,29 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
                                                unsigned* __restrict__ ilist, const double* __restrict__ x,
                                                double* __restrict__ xhold_in, int* __restrict__ live) {
   constexpr int PU = MIXED ? 9 : 6;
+  // LDS tables of the mixed test: the cell shift of each of the 27 image codes,
+  // packed as the cell-index difference it adds, and the 125 cell-offset
+  // vectors (-2..2 cells per axis) times the cell edge
+  __shared__ int img_cells[27][3];
+  __shared__ float4 off_vec[128];
+  if (MIXED) {
+    for (int t = threadIdx.x; t < 27; t += blockDim.x) {
+      img_cells[t][0] = (t % 3 - 1) * g.nc[0];
+      img_cells[t][1] = (t / 3 % 3 - 1) * g.nc[1];
+      img_cells[t][2] = (t / 9 - 1) * g.nc[2];
+    }
+    for (int t = threadIdx.x; t < 125; t += blockDim.x)
+      off_vec[t] = make_float4((float)(t % 5 - 2) * (float)(g.L[0] / g.nc[0]),
+                               (float)(t / 5 % 5 - 2) * (float)(g.L[1] / g.nc[1]),
+                               (float)(t / 25 - 2) * (float)(g.L[2] / g.nc[2]), 0.0f);
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 15;
   const int gbase = (threadIdx.x & 63) & ~15;
   // grid-stride over rows, so a step that does not re-prune launches few workgroups
   const long rstride = ((long)gridDim.x * blockDim.x) >> 4;
   const long rfirst = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 4;
   const float cut32 = (float)(cutsq_in * (1.0 + 1e-5));
-  const float cs0 = (float)(g.L[0] / g.nc[0]), cs1 = (float)(g.L[1] / g.nc[1]), cs2 = (float)(g.L[2] / g.nc[2]);
   for (long rl = rfirst; rl < R; rl += rstride) {
     const int r = (int)rl;
     const int k = row_site[r];
@@ -753,21 +769,23 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
       for (int u = 0; u < PU; u++) {
         const int jj = base + 16 * u + lane;
         const unsigned t = ent[u] >> IMG_SHIFT;
-        const int tx = (int)(t % 3u) - 1, ty = (int)((t / 3u) % 3u) - 1, tz = (int)(t / 9u) - 1;
         bool keep;
         unsigned eout = ent[u];
         if (MIXED) {
+          // the site image's cell relative to the row's cell, in -2..2 per axis
           const int cj = __float_as_int(sj32[u].w);
-          const int ox = cell_ix(cj) + tx * g.nc[0] - icx;
-          const int oy = cell_iy(cj) + ty * g.nc[1] - icy;
-          const int oz = cell_iz(cj) + tz * g.nc[2] - icz;
-          const float dx = (pk32.x - sj32[u].x) - (float)ox * cs0;
-          const float dy = (pk32.y - sj32[u].y) - (float)oy * cs1;
-          const float dz = (pk32.z - sj32[u].z) - (float)oz * cs2;
+          const int ox = cell_ix(cj) + img_cells[t][0] - icx;
+          const int oy = cell_iy(cj) + img_cells[t][1] - icy;
+          const int oz = cell_iz(cj) + img_cells[t][2] - icz;
+          const unsigned dcode = (unsigned)(ox + 5 * oy + 25 * oz + 62);
+          const float4 o = off_vec[dcode];
+          const float dx = (pk32.x - sj32[u].x) - o.x;
+          const float dy = (pk32.y - sj32[u].y) - o.y;
+          const float dz = (pk32.z - sj32[u].z) - o.z;
           keep = jj < n && dx * dx + dy * dy + dz * dz <= cut32;
-          const unsigned dcode = (unsigned)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
           eout = (ent[u] & SITE_MASK) | (dcode << MIX_SHIFT);
         } else {
+          const int tx = (int)(t % 3u) - 1, ty = (int)((t / 3u) % 3u) - 1, tz = (int)(t / 9u) - 1;
           const double dx = pk.x - (sj[u].x + (double)tx * g.L[0]);
           const double dy = pk.y - (sj[u].y + (double)ty * g.L[1]);
           const double dz = pk.z - (sj[u].z + (double)tz * g.L[2]);
