@@ -14,6 +14,11 @@ atom-equivalent steps per second in mixed precision (FP32 pair terms, FP64
 positions and sums); the FP64 path runs on the same deck for the force error
 and the speedup.  Inputs are resident in HBM before the timed region.
 
+The CAC deck (2x2x2 Gauss points on 9^3 elements) is not energy-stable
+under the reference's equations: it warms up, and its lists are rebuilt more
+often as it does, so the per-step cost grows with --steps; the default
+(20 warm-up + 200 timed steps) is the window `value` is quoted on.
+
 Prints one JSON line.  `--profile-run` instead runs only the mixed model for
 --steps steps and then a 1 GiB device-to-device copy, the calibration kernel
 of the HBM traffic counters; it is the program the committed rocprofv3
