@@ -62,21 +62,23 @@ bool step_issue(Ctx& c, double dt, int ev) {
   const int N = (int)c.ntimestep;
   c.ago++;
   const bool due = c.ago >= c.delay && c.ago % c.every == 0;
+  const bool begun = c.begun_step == N;  // the previous step's fused pass did the opening half
+  c.begun_step = -1;
   if (due && !c.check) {
-    nve_initial(c, dt);
+    if (!begun) nve_initial(c, dt);
     build_neighbor(c);
     pair_passes(c, ev, ev, Gate());
     step_end(c, dt, Gate());
     return false;
   }
   const Gate g{c.d_steps.get(), N};
-  step_begin(c, dt, N, due);
+  if (!begun) step_begin(c, dt, N, due);
   CAC_HIP(hipEventRecord(c.ev_check, c.stream));
   site_refresh(c, false, g);
   return true;
 }
 
-void step_resolve(Ctx& c, double dt, int ev) {
+void step_resolve(Ctx& c, double dt, int ev, bool fuse_next) {
   const int N = (int)c.ntimestep;
   CAC_HIP(hipEventSynchronize(c.ev_check));
   if (c.h_steps[0] == N) {
@@ -86,7 +88,12 @@ void step_resolve(Ctx& c, double dt, int ev) {
     build_inner(c);
   }
   pair_passes(c, ev, ev, Gate());
-  step_end(c, dt, Gate());
+  // Neighbor::decide of the next step, as step_issue will form it
+  const bool due_next = c.ago + 1 >= c.delay && (c.ago + 1) % c.every == 0;
+  if (fuse_next && !ev && c.check)
+    step_end_begin(c, dt, N + 1, due_next);
+  else
+    step_end(c, dt, Gate());
 }
 
 // A context about to take a new model: box, types, element type, staged
@@ -108,6 +115,7 @@ void reset_model(Ctx& c) {
   c.built = false;
   c.ago = c.ncalls = c.ndanger = 0;
   c.ntimestep = 0;
+  c.begun_step = -1;
   c.inner_builds = 0;
   c.inner_mode = -1;
   c.inner_due = 0;
@@ -794,10 +802,11 @@ int cac_hip_run(cac_hip_ctx* h, int nsteps, double dt, int thermo_every, double*
       r[4] = press;
     };
     if (th) record();
+    c.begun_step = -1;
     for (int s = 0; s < nsteps; s++) {
       const bool last = s == nsteps - 1;
       const int ev = th && ((thermo_every > 0 && (c.ntimestep + 1) % thermo_every == 0) || last) ? 1 : 0;
-      if (step_issue(c, dt, ev)) step_resolve(c, dt, ev);
+      if (step_issue(c, dt, ev)) step_resolve(c, dt, ev, !last);
       if (ev) pair_sums(c, ev, ev);
       if (ev) record();
     }

@@ -247,6 +247,7 @@ struct Ctx {
   int ago = 0, ncalls = 0, ndanger = 0;
   bool built = false;
   long ntimestep = 0;
+  long begun_step = -1;  // a step whose initial kick, drift and checks already ran (fused)
 
   // pair
   int pairstyle = 0;  // 1 lj/cut, 2 eam/alloy
@@ -304,7 +305,10 @@ int check_distance(Ctx& c, bool outer, int* inner);  // 1 when any point moved m
 void build_inner(Ctx& c);  // prune the outer list to rc + delta and compact the live sites
 // api.hip: one velocity-Verlet step in two halves (queue, then resolve the check)
 bool step_issue(Ctx& c, double dt, int ev);
-void step_resolve(Ctx& c, double dt, int ev);
+// fuse_next: the caller issues the next step right after this one and reads
+// no full-step velocities in between (the closing kick is fused with the next
+// step's opening kick, drift and checks)
+void step_resolve(Ctx& c, double dt, int ev, bool fuse_next = false);
 void reset_model(Ctx& c);  // before loading a new model into a used context
 void compact_live(Ctx& c);  // dense live-site records from the full ones (after a prune)
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
@@ -330,6 +334,9 @@ void nve_final(Ctx& c, double dt);
 // nodal force assembly + final kick in one pass
 void step_begin(Ctx& c, double dt, int step, bool outer);
 void step_end(Ctx& c, double dt, Gate g);
+// step_end of step N fused with step_begin of next_step = N + 1 (no host read
+// of the full-step velocities in between); records next_step in begun_step
+void step_end_begin(Ctx& c, double dt, int next_step, bool outer);
 void update_centers(Ctx& c);
 void nodal_assembly(Ctx& c);
 void thermo(Ctx& c, double* ke, double* temp);

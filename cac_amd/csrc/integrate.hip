@@ -254,6 +254,93 @@ void step_end(Ctx& c, double dt, Gate g) {
                                                      nodef_tab(c), mass_tab(c), dtf, g);
 }
 
+// The closing half of step N and the opening half of step N + 1 in one pass
+// over the points, for a step whose full-step velocities nobody reads (no
+// thermo output): nodal forces of every element, then per point the final
+// kick of N, the initial kick and drift of N + 1 and both displacement
+// checks of N + 1.  Each point's arithmetic is that of k_step_end followed by
+// k_step_begin, in the same order, so the state is bitwise the same.  Threads
+// [0, A) are atoms; thread A + 8e + k is node k of element e, which forms its
+// own nodal force in all three dimensions from the element's Gauss forces.
+__global__ void k_step_end_begin(const double* __restrict__ gf, double* __restrict__ f, double* __restrict__ x,
+                                 double* __restrict__ v, const int* __restrict__ ptype, const double* __restrict__ xh,
+                                 const double* __restrict__ xhi, int A, int E, int P, NodefTab t, MassTab mt,
+                                 double dtf, double dtv, double trig, double trig_in, int outer, int step,
+                                 int* __restrict__ d_steps, int* __restrict__ h_steps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int hit = 0, hit_in = 0;
+  if (p < P) {
+    double fp[3];
+    if (p < A) {
+      for (int d = 0; d < 3; d++) fp[d] = f[(size_t)d * P + p];
+    } else {
+      const int e = (p - A) / NPE, k = (p - A) - NPE * e;
+      const int E8 = NPE * E;
+      for (int d = 0; d < 3; d++) {
+        double gq[8];
+        for (int j = 0; j < 8; j++) gq[j] = gf[(size_t)d * E8 + (size_t)NPE * e + t.g2n[j]];
+        double out;
+        if (t.style == 2) {
+          out = gq[t.n2g[k]];
+        } else {
+          double fc[NPE];
+          for (int b = 0; b < NPE; b++) {
+            double sacc = 0.0;
+            for (int j = 0; j < 8; j++) sacc += t.w[j][b] * gq[j];
+            fc[b] = sacc;
+          }
+          if (t.style == 1) {
+            out = fc[k];
+          } else {
+            double sacc = 0.0;
+            for (int b = 0; b < NPE; b++) sacc += t.minv[k][b] * fc[b];
+            out = sacc;
+          }
+        }
+        f[(size_t)d * P + p] = out;
+        fp[d] = out;
+      }
+    }
+    const double dtfm = dtf / mt.m[ptype[p]];
+    double xn[3];
+    for (int d = 0; d < 3; d++) {
+      const size_t i = (size_t)d * P + p;
+      const double vh = v[i] + dtfm * fp[d];   // final kick of step N
+      const double vv = vh + dtfm * fp[d];     // initial kick of step N + 1
+      v[i] = vv;
+      xn[d] = x[i] + dtv * vv;
+      x[i] = xn[d];
+    }
+    if (outer) {
+      const double dx = xn[0] - xh[p], dy = xn[1] - xh[(size_t)P + p], dz = xn[2] - xh[2 * (size_t)P + p];
+      hit = (dx * dx + dy * dy + dz * dz > trig);
+    }
+    const double ex = xn[0] - xhi[p], ey = xn[1] - xhi[(size_t)P + p], ez = xn[2] - xhi[2 * (size_t)P + p];
+    hit_in = (ex * ex + ey * ey + ez * ez > trig_in);
+  }
+  const bool lead = (threadIdx.x & 63) == 0;
+  if (__ballot(hit) && lead) {
+    d_steps[0] = step;
+    h_steps[0] = step;
+  }
+  if (__ballot(hit_in) && lead) {
+    d_steps[1] = step;
+    h_steps[1] = step;
+  }
+}
+
+void step_end_begin(Ctx& c, double dt, int next_step, bool outer) {
+  ScopedTimer tm(c, "nve");
+  const double dtf = 0.5 * dt * c.ftm2v;
+  const double hd = 0.5 * c.inner_skin;
+  k_step_end_begin<<<grid_for(c.P, 256), 256, 0, c.stream>>>(
+      c.gf.get(), c.f.get(), c.x.get(), c.v.get(), c.ptype.get(), c.xhold.get(), c.xhold_in.get(), c.A, c.E, c.P,
+      nodef_tab(c), mass_tab(c), dtf, dt, c.triggersq, hd * hd, outer ? 1 : 0, next_step, c.d_steps.get(),
+      c.h_steps_dev);
+  c.x_dirty = true;
+  c.begun_step = next_step;
+}
+
 // ------------------------------------------------------------------ deterministic reductions
 // Each block sums a fixed slice in a fixed tree; a single block then sums the
 // block partials in order.  No atomics: the result is bitwise reproducible.

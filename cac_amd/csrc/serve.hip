@@ -284,7 +284,7 @@ void serve(cac_hip_server* s) {
       Ctx& c = s->slots[a.slot].h->c;
       const int ev = a.done + 1 == a.req->r.nsteps ? 1 : 0;
       try {
-        step_resolve(c, a.req->r.dt, ev);
+        step_resolve(c, a.req->r.dt, ev, a.done + 1 < a.req->r.nsteps);
       } catch (const HipError& e) {
         a.done = -1;
         finish(s, a, code_of(e), e.where);
