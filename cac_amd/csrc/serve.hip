@@ -1,8 +1,9 @@
 // serve.hip -- the resident serving loop of include/cac_hip.h (cac_hip_server_*).
 //
 // Independent replicas (whole models, each a Verlet::run of its own,
-// verlet.cpp:218-360) are submitted from any host thread and served by one
-// service thread that owns a fixed set of slots.  A slot is a device context
+// verlet.cpp:218-360) are submitted from any host thread and served by two
+// service threads over a fixed set of slots: a loader that admits and loads
+// replicas, and a stepper that steps the loaded ones.  A slot is a device context
 // with its own stream; it keeps its buffers from one replica to the next, so
 // serving never frees device memory (a free synchronizes the whole device).
 //
@@ -13,7 +14,7 @@
 // Stepping: every round queues the first half of a step (kick, drift, both
 // displacement checks, site refresh) for every active replica, then walks the
 // replicas again, waits for each one's check, rebuilds or re-prunes if asked
-// and queues the force passes and the closing kick.  By the time the service
+// and queues the force passes and the closing kick.  By the time the stepping
 // thread comes back to a replica its check has long finished, so the thread
 // rarely waits, and the kernels of different replicas, on different streams,
 // overlap on the device.  A replica's results do not depend on the others:
@@ -91,12 +92,14 @@ struct cac_hip_server {
   bool pair_set = false, submitted = false;
 
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_ready, cv_done;
   std::deque<std::unique_ptr<Request>> queue;
+  std::vector<Active> ready;  // loaded by the loader, not yet stepped
   std::map<long, Outcome> out;
   long next_id = 0, finished = 0;
-  bool stop = false;
-  std::thread th;
+  int in_flight = 0;  // admitted and not finished (loading, ready or stepping)
+  bool stop = false, loader_done = false;
+  std::thread th, loader;
   Clock::time_point t0;
 
   // statistics, written by the service thread under mu
@@ -196,69 +199,92 @@ void finish(cac_hip_server* s, Active& a, int status, const std::string& msg) {
     }
   }
   o.done = true;
-  sl.busy = false;
   std::lock_guard<std::mutex> lk(s->mu);
+  sl.busy = false;
+  s->in_flight--;
   if (!msg.empty()) s->err = "replica " + std::to_string(a.req->id) + ": " + msg;
   s->out[a.req->id] = std::move(o);
   s->finished++;
   s->cv_done.notify_all();
+  s->cv_work.notify_all();  // a slot is free for the loader
 }
 
+// The loader thread: admits the head of the queue into a free slot when the
+// budget allows and loads it there (upload, neighbor build, first force) on
+// the slot's own stream, while the stepping thread keeps stepping the others.
+void load_loop(cac_hip_server* s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    std::unique_ptr<Request> q;
+    int slot = -1;
+    double held = 0.0;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      for (;;) {
+        if (s->stop && s->queue.empty()) {
+          s->loader_done = true;
+          s->cv_ready.notify_all();
+          return;
+        }
+        if (!s->queue.empty()) {
+          double h = 0.0;
+          for (const Slot& sl : s->slots) h += sl.held;
+          const double need = s->queue.front()->need;
+          for (int k = 0; k < (int)s->slots.size(); k++) {
+            const Slot& sl = s->slots[k];
+            if (sl.busy) continue;
+            // a replica alone on the device is always tried
+            if (h - sl.held + std::max(sl.held, need) <= s->budget || s->in_flight == 0) {
+              slot = k;
+              break;
+            }
+          }
+          if (slot >= 0) break;
+        }
+        s->cv_work.wait(lk);
+      }
+      q = std::move(s->queue.front());
+      s->queue.pop_front();
+      Slot& sl = s->slots[slot];
+      sl.busy = true;
+      sl.held = std::max(sl.held, q->need);
+      for (const Slot& x : s->slots) held += x.held;
+      s->in_flight++;
+      s->peak_hbm = std::max(s->peak_hbm, held);
+    }
+    Active a;
+    a.slot = slot;
+    a.req = std::move(q);
+    a.t_admit = s->now();
+    std::string msg;
+    const int rc = load(s->slots[slot], *a.req, msg);
+    a.t_first = a.t_last = s->now();
+    if (rc) {
+      finish(s, a, rc, msg);
+      continue;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->ready.push_back(std::move(a));
+    s->cv_ready.notify_all();
+  }
+}
+
+// The stepping thread: takes the replicas the loader made ready, then runs
+// rounds over every active replica.
 void serve(cac_hip_server* s) {
   (void)hipSetDevice(s->device);
   std::vector<Active> act;
   std::vector<float> round_ms;
   for (;;) {
-    // admission, in arrival order
-    for (;;) {
-      std::unique_ptr<Request> q;
-      int slot = -1;
-      {
-        std::unique_lock<std::mutex> lk(s->mu);
-        if (act.empty())
-          s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
-        if (s->queue.empty()) {
-          if (act.empty() && s->stop) return;
-          break;
-        }
-        double held = 0.0;
-        for (const Slot& sl : s->slots) held += sl.held;
-        const double need = s->queue.front()->need;
-        for (int k = 0; k < (int)s->slots.size(); k++) {
-          const Slot& sl = s->slots[k];
-          if (sl.busy) continue;
-          const double after = held - sl.held + std::max(sl.held, need);
-          if (after <= s->budget || act.empty()) {  // a lone replica is always tried
-            slot = k;
-            break;
-          }
-        }
-        if (slot < 0) break;
-        q = std::move(s->queue.front());
-        s->queue.pop_front();
-      }
-      Slot& sl = s->slots[slot];
-      sl.busy = true;
-      sl.held = std::max(sl.held, q->need);
-      double held = 0.0;
-      for (const Slot& x : s->slots) held += x.held;
-      std::string msg;
-      Active a;
-      a.slot = slot;
-      a.req = std::move(q);
-      a.t_admit = s->now();
-      const int rc = load(sl, *a.req, msg);
-      a.t_first = a.t_last = s->now();
-      if (rc) {
-        finish(s, a, rc, msg);
-        continue;
-      }
-      act.push_back(std::move(a));
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->peak_hbm = std::max(s->peak_hbm, held);
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (act.empty())
+        s->cv_ready.wait(lk, [&] { return !s->ready.empty() || (s->loader_done && s->in_flight == 0); });
+      for (Active& a : s->ready) act.push_back(std::move(a));
+      s->ready.clear();
+      if (act.empty()) return;  // the loader has finished and nothing is in flight
       s->peak_active = std::max(s->peak_active, (int)act.size());
     }
-    if (act.empty()) continue;
     round_ms.clear();
     // one round: queue the first half of a step everywhere, then resolve
     for (Active& a : act) {
@@ -339,6 +365,7 @@ int cac_hip_server_create(cac_hip_server** out, int device, int slots, double hb
   }
   s->budget = hbm_budget_bytes;
   s->th = std::thread(serve, s);
+  s->loader = std::thread(load_loop, s);
   *out = s;
   return CAC_HIP_OK;
 }
@@ -350,6 +377,8 @@ int cac_hip_server_destroy(cac_hip_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
+  s->cv_ready.notify_all();
+  if (s->loader.joinable()) s->loader.join();
   if (s->th.joinable()) s->th.join();
   for (Slot& sl : s->slots) cac_hip_destroy(sl.h);
   delete s;

@@ -145,8 +145,9 @@ int cac_hip_set_inner_skin(cac_hip_ctx *ctx, double delta);
  * the reference, whose one process runs one model; each replica is a whole
  * Verlet::run of its own model, verlet.cpp:218-360).
  *
- * The server owns `slots` device contexts, each with its own stream, and one
- * service thread.  Submitted replicas wait in arrival order and are admitted
+ * The server owns `slots` device contexts, each with its own stream, and two
+ * service threads: a loader (admission, upload, neighbor build, first force)
+ * and a stepper.  Submitted replicas wait in arrival order and are admitted
  * when a slot is free and the HBM the replica needs (estimated from its
  * atoms, elements and unit cells) fits the budget beside what the slots
  * hold.  Admitted replicas are stepped round-robin: each round queues the

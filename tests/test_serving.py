@@ -8,6 +8,7 @@ import pytest
 from cac_amd import decks
 
 TAB = decks.cu_eam_tables()
+NSTEP = 150
 
 
 def test_serving_stream_follows_its_distributions():
@@ -44,11 +45,11 @@ def _specs(n, seed):
 
 @pytest.mark.gpu
 def test_replicas_same_alone_and_in_the_mix():
-    """Six replicas of uneven size, temperature and strain, 60 mixed steps each:
-    run alone through the resident loop, then together on three slots (so
-    slots are reused by replicas of other sizes), then one at a time under a
-    budget that admits a single replica: final positions bitwise equal, final
-    energies equal."""
+    """Six replicas of uneven size, temperature and strain, 150 mixed steps
+    each: run alone through the resident loop, then together on three slots
+    (so slots are reused by replicas of other sizes, and replicas overlap on
+    the device), then one at a time under a budget that admits a single
+    replica: final positions bitwise equal, final energies equal."""
     import cac_amd
     specs = _specs(6, 11)
     dks = [decks.serving_deck(s) for s in specs]
@@ -56,19 +57,22 @@ def test_replicas_same_alone_and_in_the_mix():
     for d in dks:
         m = cac_amd.Model(d, "eam", skin=2.0, eam_tables=TAB, precision=cac_amd.MIXED)
         m.setup()
-        th = m.run(60, 0.001)
+        th = m.run(NSTEP, 0.001)
         alone.append((m.atoms()[0], m.elements()["nodex"] if d.nelements else None, th[-1]))
         m.close()
     for slots, budget in ((3, 0.0), (2, 1.0)):
         srv = cac_amd.Server(slots=slots, hbm_budget_bytes=budget, eam_tables=TAB)
-        ids = [srv.submit(d, 60, 0.001, skin=2.0, keep_state=True) for d in dks]
+        ids = [srv.submit(d, NSTEP, 0.001, skin=2.0, keep_state=True) for d in dks]
         srv.drain()
         st = srv.stats()
-        assert st["steps"] == 6 * 60
-        assert st["peak_active"] == (3 if budget == 0.0 else 1), st
+        assert st["steps"] == 6 * NSTEP
+        if budget == 0.0:
+            assert 2 <= st["peak_active"] <= slots, st  # replicas overlapped
+        else:
+            assert st["peak_active"] == 1, st
         for rid, d, (x0, nx0, th0) in zip(ids, dks, alone):
             r = srv.result(rid)
-            assert r["status"] == 0 and r["nsteps"] == 60
+            assert r["status"] == 0 and r["nsteps"] == NSTEP
             x, nx = srv.state(rid, d)
             assert np.array_equal(x, x0)
             if d.nelements:
