@@ -96,33 +96,38 @@ def nve_drift(cac_amd, deck, tab, precision, steps, every=100):
             "e0": float(e[0]), "e_end": float(e[-1])}
 
 
-def serve_run(cac_amd, decks, tab, n, steps, rate, slots, seed=77):
+def serve_run(cac_amd, decks, tab, n, steps, rates, slots, seed=77):
     """The serving workload: ``n`` replica requests (sizes log-uniform 2k-200k
     atom-eq, core/coarse 0.05-0.3, 100-600 K, +-2 % uniaxial strain) arriving
-    as a Poisson stream at ``rate`` per second, each running ``steps`` mixed
-    steps on the resident server.  Requests are submitted at their arrival
-    times; decks are built before the clock starts."""
+    as a Poisson stream, each running ``steps`` mixed steps on the resident
+    server; one run per arrival rate in ``rates`` (per second), over the same
+    replicas.  Requests are submitted at their arrival times; decks are built
+    before the clock starts."""
     import numpy as np
-    specs = decks.serving_stream(n=n, seed=seed, rate=rate)
+    specs = decks.serving_stream(n=n, seed=seed, rate=1.0)
     dks = [decks.serving_deck(sp) for sp in specs]
-    srv = cac_amd.Server(slots=slots, eam_tables=tab)
-    t0 = time.perf_counter()
-    ids = []
-    for sp, d in zip(specs, dks):
-        wait = sp["arrival"] - (time.perf_counter() - t0)
-        if wait > 0:
-            time.sleep(wait)
-        ids.append(srv.submit(d, steps, DT, skin=SKIN, precision=cac_amd.MIXED))
-    srv.drain()
-    wall = time.perf_counter() - t0
-    res = [srv.result(i) for i in ids]
-    st = srv.stats()
-    err = srv.L.cac_hip_server_last_error(srv.h).decode()
-    srv.close()
-    q = np.array([r["t_admit"] - r["t_submit"] for r in res])
-    mean_step = np.array([r["step_ms_mean"] for r in res])
-    ae = np.array([r["atom_equivalents"] for r in res], dtype=float)
-    return {"replicas": n, "steps_each": steps, "arrival_rate_per_s": rate, "slots": slots,
+    out = {}
+    for rate in rates:
+        arrive = [sp["arrival"] / rate for sp in specs]  # the unit-rate stream, rescaled
+        srv = cac_amd.Server(slots=slots, eam_tables=tab)
+        t0 = time.perf_counter()
+        ids = []
+        for t_arr, d in zip(arrive, dks):
+            wait = t_arr - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            ids.append(srv.submit(d, steps, DT, skin=SKIN, precision=cac_amd.MIXED))
+        srv.drain()
+        wall = time.perf_counter() - t0
+        res = [srv.result(i) for i in ids]
+        st = srv.stats()
+        err = srv.L.cac_hip_server_last_error(srv.h).decode()
+        srv.close()
+        q = np.array([r["t_admit"] - r["t_submit"] for r in res])
+        mean_step = np.array([r["step_ms_mean"] for r in res])
+        ae = np.array([r["atom_equivalents"] for r in res], dtype=float)
+        out[f"rate_{rate:g}_per_s"] = {
+            "replicas": n, "steps_each": steps, "arrival_rate_per_s": rate, "slots": slots,
             "failed": int(sum(r["status"] != 0 for r in res)), "last_error": err or None,
             "atom_eq_mean": float(ae.mean()), "atom_eq_range": [float(ae.min()), float(ae.max())],
             "step_ms_p50": st["p50_ms"], "step_ms_p99": st["p99_ms"],
@@ -131,6 +136,7 @@ def serve_run(cac_amd, decks, tab, n, steps, rate, slots, seed=77):
             "queue_wait_s_p50": float(np.percentile(q, 50)), "queue_wait_s_p99": float(np.percentile(q, 99)),
             "atom_steps_per_s": st["atom_steps_per_s"], "wall_s": wall, "peak_active": st["peak_active"],
             "peak_hbm_gib_estimate": st["peak_hbm_bytes"] / 2 ** 30}
+    return out
 
 
 def force_bytes(P, inner_mean, live_sites):
@@ -189,7 +195,9 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=250,
                     help="steps per replica: CAC with 2x2x2 Gauss points on 9^3 elements is unstable past "
                          "about 450 steps (element deformed more than 2x, in the oracle as on the GPU)")
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate (replicas per second)")
+    ap.add_argument("--serve-rates", default="100,400",
+                    help="Poisson arrival rates (replicas per second), comma-separated: a light load for latency "
+                         "and a saturating one for capacity")
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--profile-run", action="store_true",
@@ -205,11 +213,12 @@ def main():
     import cac_amd
     from cac_amd import decks
 
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
     build_deck, desc = WORKLOADS[args.workload]
     tab = decks.cu_eam_tables()
     if args.serve_only:
         print(json.dumps({"serving": serve_run(cac_amd, decks, tab, args.serve_replicas, args.serve_steps,
-                                               args.serve_rate, args.serve_slots)}))
+                                               rates, args.serve_slots)}))
         return
     deck = build_deck(decks)
     P = deck.atom_equivalents
@@ -288,8 +297,7 @@ def main():
 
     serving = None
     if args.serve_replicas > 0:
-        serving = serve_run(cac_amd, decks, tab, args.serve_replicas, args.serve_steps, args.serve_rate,
-                            args.serve_slots)
+        serving = serve_run(cac_amd, decks, tab, args.serve_replicas, args.serve_steps, rates, args.serve_slots)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -64,6 +65,7 @@ struct Slot {
 struct Active {
   std::unique_ptr<Request> req;
   int slot = -1;
+  int stepper = -1;  // the stepping thread that owns it
   int done = 0;
   double t_admit = 0.0, t_first = 0.0, t_last = 0.0;
   bool issued = false;
@@ -94,12 +96,14 @@ struct cac_hip_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_ready, cv_done;
   std::deque<std::unique_ptr<Request>> queue;
-  std::vector<Active> ready;  // loaded by the loader, not yet stepped
+  std::vector<std::vector<Active>> ready;  // per stepping thread: loaded, not yet stepped
+  std::vector<int> load_of;                // per stepping thread: replicas assigned
   std::map<long, Outcome> out;
   long next_id = 0, finished = 0;
   int in_flight = 0;  // admitted and not finished (loading, ready or stepping)
   bool stop = false, loader_done = false;
-  std::thread th, loader;
+  std::vector<std::thread> steppers;
+  std::thread loader;
   Clock::time_point t0;
 
   // statistics, written by the service thread under mu
@@ -202,6 +206,7 @@ void finish(cac_hip_server* s, Active& a, int status, const std::string& msg) {
   std::lock_guard<std::mutex> lk(s->mu);
   sl.busy = false;
   s->in_flight--;
+  if (a.stepper >= 0) s->load_of[a.stepper]--;
   if (!msg.empty()) s->err = "replica " + std::to_string(a.req->id) + ": " + msg;
   s->out[a.req->id] = std::move(o);
   s->finished++;
@@ -264,14 +269,20 @@ void load_loop(cac_hip_server* s) {
       continue;
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    s->ready.push_back(std::move(a));
+    int best = 0;  // the stepping thread with the fewest replicas takes it
+    for (int k = 1; k < (int)s->load_of.size(); k++)
+      if (s->load_of[k] < s->load_of[best]) best = k;
+    a.stepper = best;
+    s->load_of[best]++;
+    s->ready[best].push_back(std::move(a));
     s->cv_ready.notify_all();
   }
 }
 
-// The stepping thread: takes the replicas the loader made ready, then runs
-// rounds over every active replica.
-void serve(cac_hip_server* s) {
+// A stepping thread: takes the replicas the loader assigned to it, then runs
+// rounds over them.  Several stepping threads share the host-side cost of
+// queueing the launches; each owns its replicas' slots while it steps them.
+void serve(cac_hip_server* s, int me) {
   (void)hipSetDevice(s->device);
   std::vector<Active> act;
   std::vector<float> round_ms;
@@ -279,11 +290,11 @@ void serve(cac_hip_server* s) {
     {
       std::unique_lock<std::mutex> lk(s->mu);
       if (act.empty())
-        s->cv_ready.wait(lk, [&] { return !s->ready.empty() || (s->loader_done && s->in_flight == 0); });
-      for (Active& a : s->ready) act.push_back(std::move(a));
-      s->ready.clear();
+        s->cv_ready.wait(lk, [&] { return !s->ready[me].empty() || (s->loader_done && s->in_flight == 0); });
+      for (Active& a : s->ready[me]) act.push_back(std::move(a));
+      s->ready[me].clear();
       if (act.empty()) return;  // the loader has finished and nothing is in flight
-      s->peak_active = std::max(s->peak_active, (int)act.size());
+      s->peak_active = std::max(s->peak_active, s->in_flight);
     }
     round_ms.clear();
     // one round: queue the first half of a step everywhere, then resolve
@@ -364,7 +375,12 @@ int cac_hip_server_create(cac_hip_server** out, int device, int slots, double hb
     hbm_budget_bytes = 0.75 * (double)fr;
   }
   s->budget = hbm_budget_bytes;
-  s->th = std::thread(serve, s);
+  int nstep = 4;
+  if (const char* e = std::getenv("CAC_HIP_SERVE_STEPPERS")) nstep = std::max(1, std::min(16, std::atoi(e)));
+  nstep = std::min(nstep, slots);
+  s->ready.resize(nstep);
+  s->load_of.assign(nstep, 0);
+  for (int k = 0; k < nstep; k++) s->steppers.emplace_back(serve, s, k);
   s->loader = std::thread(load_loop, s);
   *out = s;
   return CAC_HIP_OK;
@@ -379,7 +395,8 @@ int cac_hip_server_destroy(cac_hip_server* s) {
   s->cv_work.notify_all();
   s->cv_ready.notify_all();
   if (s->loader.joinable()) s->loader.join();
-  if (s->th.joinable()) s->th.join();
+  for (std::thread& t : s->steppers)
+    if (t.joinable()) t.join();
   for (Slot& sl : s->slots) cac_hip_destroy(sl.h);
   delete s;
   return CAC_HIP_OK;
