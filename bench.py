@@ -187,7 +187,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--inner-skin", type=float, default=-1.0,
-                    help="prune margin delta of the inner list (A); < 0 takes 0.35 * skin")
+                    help="prune margin delta of the inner list (A); < 0 takes 0.3 * skin")
     ap.add_argument("--drift-steps", type=int, default=10000,
                     help="NVE steps of the FP64-vs-mixed drift comparison on the 32,000-atom EAM block (0: skip)")
     ap.add_argument("--serve-replicas", type=int, default=512,

@@ -214,7 +214,7 @@ void neighbor_init(Ctx& c) {  // neighbor.cpp:241-283 with one cutoff
   c.cutneigh = c.cutforce + (c.cutforce > 0.0 ? c.skin : 0.0);
   c.cutneighsq = c.cutneigh * c.cutneigh;
   c.triggersq = 0.25 * c.skin * c.skin;
-  c.inner_skin = c.inner_skin_set < 0.0 ? 0.35 * c.skin : std::min(c.inner_skin_set, c.skin);
+  c.inner_skin = c.inner_skin_set < 0.0 ? 0.3 * c.skin : std::min(c.inner_skin_set, c.skin);
 }
 
 int require_model(Ctx& c) {

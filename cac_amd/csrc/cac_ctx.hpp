@@ -231,7 +231,7 @@ struct Ctx {
   DBuf<int> innum, ilist;
   int icap = 0;
   double inner_skin = 0.0;       // delta in use
-  double inner_skin_set = -1.0;  // requested delta; < 0 picks 0.35 skin
+  double inner_skin_set = -1.0;  // requested delta; < 0 picks 0.3 skin
   double inner_cutsq = 0.0;
   int inner_mode = -1;       // precision the inner list was built for (-1: stale)
   long inner_builds = 0;

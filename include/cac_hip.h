@@ -137,7 +137,7 @@ int cac_hip_neigh_stats(cac_hip_ctx *ctx, int *ncalls, int *ndanger);
  * counterpart in the reference; results do not depend on it).  Reports the
  * prune count, mean inner entries per point, delta, and the number of sites
  * the inner lists reference (each NULL-able); delta < 0 on input to
- * cac_hip_set_inner_skin restores the default 0.35 * skin. */
+ * cac_hip_set_inner_skin restores the default 0.3 * skin. */
 int cac_hip_inner_stats(cac_hip_ctx *ctx, long *builds, double *mean_entries, double *delta, int *live_sites);
 int cac_hip_set_inner_skin(cac_hip_ctx *ctx, double delta);
 
