@@ -295,7 +295,8 @@ class Model:
         return arr[order]
 
     def enable_timing(self, on=True):
-        self._chk(self.L.cac_hip_enable_timing(self.h, 1 if on else 0))
+        """True / 1: time every kernel section; 2: only the pair force pass; False / 0: off."""
+        self._chk(self.L.cac_hip_enable_timing(self.h, int(on)))
 
     def kernel_time(self, name):
         a, n = C.c_double(), C.c_long()

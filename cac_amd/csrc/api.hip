@@ -24,28 +24,54 @@ Ctx::~Ctx() {
   if (h_steps) (void)hipHostFree(const_cast<int*>(h_steps));
   if (h_stage) (void)hipHostFree(h_stage);
   if (ev_check) (void)hipEventDestroy(ev_check);
-  for (auto& row : ev)
-    for (auto& e : row)
-      if (e) (void)hipEventDestroy(e);
+  for (auto& e : ev_pool)
+    if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+static hipEvent_t take_event(Ctx& c) {
+  if (c.ev_used == c.ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    CAC_HIP(hipEventCreate(&e));
+    c.ev_pool.push_back(e);
+  }
+  return c.ev_pool[c.ev_used++];
+}
+
+void resolve_timers(Ctx& c) {
+  if (c.ev_pending.empty()) {
+    c.ev_used = 0;
+    return;
+  }
+  CAC_HIP(hipStreamSynchronize(c.stream));
+  for (const auto& p : c.ev_pending) {
+    float ms = 0.0f;
+    CAC_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
+    Timer& t = c.timers[p.name];
+    t.sum_us += 1000.0 * ms;
+    t.count++;
+  }
+  c.ev_pending.clear();
+  c.ev_used = 0;
+}
+
+static bool timed_section(const Ctx& c, const char* nm) {
+  if (c.timing == 1) return true;
+  return c.timing == 2 && (std::strcmp(nm, "eam_force") == 0 || std::strcmp(nm, "lj_force") == 0);
+}
+
 ScopedTimer::ScopedTimer(Ctx& cc, const char* nm) : c(cc), name(nm) {
-  if (!c.timing || c.tdepth >= 8) return;
-  depth = c.tdepth++;
-  CAC_HIP(hipEventRecord(c.ev[0][depth], c.stream));
+  if (!timed_section(c, nm)) return;
+  if (c.ev_pending.size() >= 512) resolve_timers(c);
+  start = take_event(c);
+  stop = take_event(c);
+  CAC_HIP(hipEventRecord(start, c.stream));
 }
 
 ScopedTimer::~ScopedTimer() {
-  if (depth < 0) return;
-  c.tdepth = depth;
-  if (hipEventRecord(c.ev[1][depth], c.stream) != hipSuccess) return;
-  if (hipEventSynchronize(c.ev[1][depth]) != hipSuccess) return;
-  float ms = 0.0f;
-  if (hipEventElapsedTime(&ms, c.ev[0][depth], c.ev[1][depth]) != hipSuccess) return;
-  Timer& t = c.timers[name];
-  t.sum_us += 1000.0 * ms;
-  t.count++;
+  if (!start) return;
+  if (hipEventRecord(stop, c.stream) != hipSuccess) return;
+  c.ev_pending.push_back({name, start, stop});
 }
 
 // One velocity-Verlet step (verlet.cpp:218-360), in two halves so that a
@@ -278,8 +304,6 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     c.device = device;
     CAC_HIP(hipSetDevice(device));
     CAC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    for (auto& row : c.ev)
-      for (auto& e : row) CAC_HIP(hipEventCreate(&e));
     CAC_HIP(hipHostMalloc((void**)&c.h_red, sizeof(double) * 64, hipHostMallocDefault));
     CAC_HIP(hipHostMalloc((void**)&c.h_flags, sizeof(ErrorFlags), hipHostMallocDefault));
     c.d_flags.reserve(1);
@@ -936,7 +960,9 @@ int cac_hip_sync(cac_hip_ctx* h) {
 
 int cac_hip_enable_timing(cac_hip_ctx* h, int on) {
   return guarded(h, [&](Ctx& c) -> int {
-    c.timing = on != 0;
+    if (on < 0 || on > 2) return fail(h, CAC_HIP_EARG, "timing mode must be 0, 1 or 2");
+    resolve_timers(c);
+    c.timing = on;
     c.timers.clear();
     return CAC_HIP_OK;
   });
@@ -944,6 +970,7 @@ int cac_hip_enable_timing(cac_hip_ctx* h, int on) {
 
 int cac_hip_kernel_time(cac_hip_ctx* h, const char* name, double* avg_us, long* launches) {
   return guarded(h, [&](Ctx& c) -> int {
+    resolve_timers(c);
     auto it = c.timers.find(name ? name : "");
     if (it == c.timers.end()) {
       if (avg_us) *avg_us = 0.0;

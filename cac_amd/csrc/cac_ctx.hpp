@@ -287,11 +287,18 @@ struct Ctx {
   size_t h_stage_n = 0;
   DBuf<ErrorFlags> d_flags;
 
-  // timing
-  bool timing = false;
+  // timing: an event pair on the context stream around each timed section,
+  // read back only when the times are asked for (or the pool is full), so a
+  // timed run is not synchronized section by section
+  int timing = 0;  // 0 off, 1 every section, 2 the pair force pass only
   std::map<std::string, Timer> timers;
-  hipEvent_t ev[2][8] = {};  // start / stop events of nested timers, by depth
-  int tdepth = 0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct PendingTime {
+    const char* name;
+    hipEvent_t start, stop;
+  };
+  std::vector<PendingTime> ev_pending;
 
   ~Ctx();
 };
@@ -343,13 +350,15 @@ void thermo(Ctx& c, double* ke, double* temp);
 void reduce_rows(Ctx& c, const double* data, int n, int ncomp, double* out);
 
 // timing helpers (api.hip)
-struct ScopedTimer {  // GPU time between two events on the context stream; nests up to 8 deep
+struct ScopedTimer {  // GPU time between two events on the context stream; sections may nest
   Ctx& c;
   const char* name;
-  int depth = -1;
+  hipEvent_t start = nullptr, stop = nullptr;
   ScopedTimer(Ctx& cc, const char* nm);
   ~ScopedTimer();
 };
+// waits for the stream and adds every recorded section to its timer
+void resolve_timers(Ctx& c);
 
 inline int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
 

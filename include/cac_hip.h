@@ -124,10 +124,14 @@ int cac_hip_download_elements(cac_hip_ctx *ctx, double *nodex, double *nodev, do
 int cac_hip_download_rho(cac_hip_ctx *ctx, double *atomrho, double *noderho);
 int cac_hip_counts(cac_hip_ctx *ctx, int *natoms, int *nelements, int *npoints);
 int cac_hip_sync(cac_hip_ctx *ctx);
-/* Timings of the last cac_hip_run: average microseconds per launch of the named
- * kernel group ("eam_density", "eam_embed", "eam_force", "lj_force", "site_refresh",
- * "nodef", "nve", "neigh_build"), and the launch count; measured with hipEvents
- * on the context's stream when timing is enabled. */
+/* Section timings since timing was last enabled: average microseconds per
+ * launch of the named kernel group ("eam_density", "eam_embed", "eam_force",
+ * "lj_force", "site_refresh", "nodef", "nve", "neigh_build", "inner_build") and
+ * the launch count, from a hipEvent pair recorded on the context's stream
+ * around each section.  The events are read back only by cac_hip_kernel_time
+ * (or when 512 sections are pending), so a timed run keeps its overlap.
+ * on: 0 off, 1 every section, 2 only the pair force pass ("eam_force",
+ * "lj_force"), cheap enough to leave on during a measured run. */
 int cac_hip_enable_timing(cac_hip_ctx *ctx, int on);
 int cac_hip_kernel_time(cac_hip_ctx *ctx, const char *name, double *avg_us, long *launches);
 /* neighbor list build / rebuild statistics: builds, dangerous builds (neighbor.cpp:1681) */

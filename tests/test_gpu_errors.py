@@ -111,3 +111,28 @@ def test_capacity_grows_without_changing_lists():
     assert np.array_equal(g.full_sets(), ref)
     assert g.neigh_info()["capacity"] >= rows.max()
     g.close()
+
+
+def test_section_timing_modes():
+    """Timing mode 2 times only the pair force pass, one event pair per launch
+    read back after the run; mode 1 times every section; other modes are an
+    argument error."""
+    import cac_amd
+    g = cac_amd.Model(small_cac(), "eam", skin=2.0, eam_tables=TAB, precision=cac_amd.MIXED)
+    g.setup()
+    g.enable_timing(2)
+    g.run(12, 0.001, thermo=False)
+    us, n = g.kernel_time("eam_force")
+    assert n == 12 and us > 0.0
+    assert g.kernel_time("eam_density") == (0.0, 0)
+    g.enable_timing(1)
+    g.run(3, 0.001, thermo=False)
+    for name in ("eam_density", "eam_embed", "eam_force"):
+        us, n = g.kernel_time(name)
+        assert n == 3 and us > 0.0, name
+    us, n = g.kernel_time("site_refresh")
+    assert n >= 3 and us > 0.0
+    with pytest.raises(cac_amd.CacHipError):
+        g.enable_timing(3)
+    g.enable_timing(0)
+    g.close()
