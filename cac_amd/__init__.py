@@ -115,6 +115,7 @@ def lib():
             "cac_hip_server_state": [vp, C.c_long, vp, vp],
             "cac_hip_server_stats": [vp, C.POINTER(d), C.POINTER(d), C.POINTER(C.c_long), C.POINTER(d),
                                      C.POINTER(i), C.POINTER(d)],
+            "cac_hip_server_segments": [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -374,8 +375,11 @@ class Server:
         n, peak = C.c_long(), C.c_int()
         self._chk(self.L.cac_hip_server_stats(self.h, C.byref(p50), C.byref(p99), C.byref(n), C.byref(rate),
                                               C.byref(peak), C.byref(hbm)))
+        sl, ss = C.c_long(), C.c_long()
+        self._chk(self.L.cac_hip_server_segments(self.h, C.byref(sl), C.byref(ss)))
         return dict(p50_ms=p50.value, p99_ms=p99.value, steps=n.value, atom_steps_per_s=rate.value,
-                    peak_active=peak.value, peak_hbm_bytes=hbm.value)
+                    peak_active=peak.value, peak_hbm_bytes=hbm.value, segmented_launches=sl.value,
+                    segmented_steps=ss.value)
 
     def close(self):
         if getattr(self, "h", None):

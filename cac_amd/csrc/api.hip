@@ -24,6 +24,7 @@ Ctx::~Ctx() {
   if (h_steps) (void)hipHostFree(const_cast<int*>(h_steps));
   if (h_stage) (void)hipHostFree(h_stage);
   if (ev_check) (void)hipEventDestroy(ev_check);
+  if (ev_seg) (void)hipEventDestroy(ev_seg);
   for (auto& e : ev_pool)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -104,7 +105,9 @@ bool step_issue(Ctx& c, double dt, int ev) {
   return true;
 }
 
-void step_resolve(Ctx& c, double dt, int ev, bool fuse_next) {
+// step_resolve in three parts, so that a caller can run the pair passes of
+// several models together between the first and the last
+void step_prepare(Ctx& c) {
   const int N = (int)c.ntimestep;
   CAC_HIP(hipEventSynchronize(c.ev_check));
   if (c.h_steps[0] == N) {
@@ -113,13 +116,23 @@ void step_resolve(Ctx& c, double dt, int ev, bool fuse_next) {
   } else if (c.h_steps[1] == N) {
     build_inner(c);
   }
-  pair_passes(c, ev, ev, Gate());
+  ensure_inner(c);
+}
+
+void step_close(Ctx& c, double dt, int ev, bool fuse_next) {
+  const int N = (int)c.ntimestep;
   // Neighbor::decide of the next step, as step_issue will form it
   const bool due_next = c.ago + 1 >= c.delay && (c.ago + 1) % c.every == 0;
   if (fuse_next && !ev && c.check)
     step_end_begin(c, dt, N + 1, due_next);
   else
     step_end(c, dt, Gate());
+}
+
+void step_resolve(Ctx& c, double dt, int ev, bool fuse_next) {
+  step_prepare(c);
+  pair_passes(c, ev, ev, Gate());
+  step_close(c, dt, ev, fuse_next);
 }
 
 // A context about to take a new model: box, types, element type, staged
@@ -319,6 +332,7 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     c.h_steps = hs;
     CAC_HIP(hipHostGetDevicePointer((void**)&c.h_steps_dev, hs, 0));
     CAC_HIP(hipEventCreateWithFlags(&c.ev_check, hipEventDisableTiming));
+    CAC_HIP(hipEventCreateWithFlags(&c.ev_seg, hipEventDisableTiming));
     c.units = units;
     if (const char* e = std::getenv("CAC_HIP_TPR")) c.tpr = std::atoi(e);
     if (const char* e = std::getenv("CAC_HIP_TPR_FORCE")) c.tpr_force = std::atoi(e);

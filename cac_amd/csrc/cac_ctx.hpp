@@ -283,6 +283,7 @@ struct Ctx {
   volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
   int* h_steps_dev = nullptr;       // device address of h_steps
   hipEvent_t ev_check = nullptr;    // recorded after the check of a speculative step
+  hipEvent_t ev_seg = nullptr;      // recorded when the model is ready for segmented pair passes
   double* h_stage = nullptr;        // pinned staging of the per-step host handover
   size_t h_stage_n = 0;
   DBuf<ErrorFlags> d_flags;
@@ -316,6 +317,10 @@ bool step_issue(Ctx& c, double dt, int ev);
 // no full-step velocities in between (the closing kick is fused with the next
 // step's opening kick, drift and checks)
 void step_resolve(Ctx& c, double dt, int ev, bool fuse_next = false);
+// step_resolve = step_prepare (wait for the check, rebuild or re-prune), the
+// pair passes, step_close (nodal forces and the closing kick)
+void step_prepare(Ctx& c);
+void step_close(Ctx& c, double dt, int ev, bool fuse_next);
 void reset_model(Ctx& c);  // before loading a new model into a used context
 void compact_live(Ctx& c);  // dense live-site records from the full ones (after a prune)
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
@@ -332,6 +337,13 @@ void order_rows(Ctx& c);      // rows in Morton order of their cells
 void site_refresh(Ctx& c, bool full, Gate g = Gate());
 void pair_compute(Ctx& c, int eflag, int vflag);      // passes + nodal forces + energy/virial sums
 void pair_passes(Ctx& c, int eflag, int vflag, Gate g);  // the pair passes alone
+void ensure_inner(Ctx& c);                               // re-prune if precision or delta changed
+// the pair passes (no energy / virial) of up to max_segments() models that
+// segmentable() accepts and that share one potential, one launch per pass on
+// `stream`; the caller orders the models' streams around it
+bool segmentable(const Ctx& c);
+int max_segments();
+void pair_passes_segmented(Ctx* const* cs, int n, hipStream_t stream);
 void pair_sums(Ctx& c, int eflag, int vflag);            // energy / virial of the last passes
 
 // integrate.hip

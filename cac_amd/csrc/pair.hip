@@ -403,6 +403,64 @@ __device__ __forceinline__ void walk_row(const unsigned* __restrict__ base, unsi
 
 // ------------------------------------------------------------------ EAM density (pair_eam.cpp:721-813)
 
+// Density of the rows of chunk ch (NT / TPR rows, TPR lanes each) of one
+// model.  offs holds the model's cell-offset vectors (mixed path); the spline
+// table is staged in LDS by the caller.
+template <int TPR, bool MIXED, int NT, int U>
+__device__ __forceinline__ void density_chunk(const PairArgs& a, long ch, const float4* offs, const float4* tab32,
+                                              const double4* lds64, double rdr, int nr,
+                                              double* __restrict__ rho_out) {
+  const int lane = threadIdx.x % TPR;
+  const int rsub = threadIdx.x / TPR;
+  const float rdr32 = (float)rdr;
+  const float cut32 = (float)a.cutsq;
+  const int r = (int)(ch * (NT / TPR)) + rsub;
+  double acc = 0.0;
+  if (r < a.R) {
+    const int k = a.row_site[r];
+    const int n = a.nnum[r];
+    const unsigned row0 = (unsigned)r * (unsigned)a.cap;
+    if (MIXED) {
+      const float4 pi = a.site32[k];
+      walk_row<TPR, U, float4>(
+          a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
+          [&](unsigned ent, const float4& sj, bool valid) {
+            const float4 o = offs[ent >> MIX_SHIFT];
+            const float dx = (pi.x - sj.x) - o.x;
+            const float dy = (pi.y - sj.y) - o.y;
+            const float dz = (pi.z - sj.z) - o.z;
+            const float rsq = dx * dx + dy * dy + dz * dz;
+            // entries past the row's end or beyond rc count 0 by a mask
+            // multiply: a select would become a branch per entry
+            const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
+            float p;
+            const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, nr, p);
+            const float4 c = tab32[m];
+            acc += (double)(okf * (((c.x * p + c.y) * p + c.z) * p + c.w));
+          });
+    } else {
+      const double4 pi = a.site[k];
+      walk_row<TPR, U, double4>(
+          a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
+          [&](unsigned ent, const double4& sj, bool valid) {
+            double shx, shy, shz;
+            img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
+            const double delx = pi.x - (sj.x + shx);
+            const double dely = pi.y - (sj.y + shy);
+            const double delz = pi.z - (sj.z + shz);
+            const double rsq = delx * delx + dely * dely + delz * delz;
+            const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
+            double p;
+            const int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
+            const double4 c = lds64[m];
+            acc += okf * (((c.x * p + c.y) * p + c.z) * p + c.w);
+          });
+    }
+  }
+  acc = row_sum<TPR>(acc);
+  if (lane == 0 && r < a.R) rho_out[a.row_point[r]] = acc;
+}
+
 // the density pass stages only the rho(r) half of the table (knot m at m)
 template <int TPR, bool MIXED, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __restrict__ tab, double rdr, int nr,
@@ -418,58 +476,9 @@ __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __re
   } else {
     stage_table_strided(lds64, static_cast<const double4*>(tab), nr + 1);
   }
-  const int lane = threadIdx.x % TPR;
-  const int rsub = threadIdx.x / TPR;
   const Sched s = sched(a.nchunks);
-  const float rdr32 = (float)rdr;
-  const float cut32 = (float)a.cutsq;
-  for (long ch = s.first; ch < s.end; ch += s.stride) {
-    const int r = (int)(ch * (NT / TPR)) + rsub;
-    double acc = 0.0;
-    if (r < a.R) {
-      const int k = a.row_site[r];
-      const int n = a.nnum[r];
-      const unsigned row0 = (unsigned)r * (unsigned)a.cap;
-      if (MIXED) {
-        const float4 pi = a.site32[k];
-        walk_row<TPR, U, float4>(
-            a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
-            [&](unsigned ent, const float4& sj, bool valid) {
-              const float4 o = offs[ent >> MIX_SHIFT];
-              const float dx = (pi.x - sj.x) - o.x;
-              const float dy = (pi.y - sj.y) - o.y;
-              const float dz = (pi.z - sj.z) - o.z;
-              const float rsq = dx * dx + dy * dy + dz * dz;
-              // entries past the row's end or beyond rc count 0 by a mask
-              // multiply: a select would become a branch per entry
-              const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
-              float p;
-              const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, nr, p);
-              const float4 c = tab32[m];
-              acc += (double)(okf * (((c.x * p + c.y) * p + c.z) * p + c.w));
-            });
-      } else {
-        const double4 pi = a.site[k];
-        walk_row<TPR, U, double4>(
-            a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
-            [&](unsigned ent, const double4& sj, bool valid) {
-              double shx, shy, shz;
-              img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
-              const double delx = pi.x - (sj.x + shx);
-              const double dely = pi.y - (sj.y + shy);
-              const double delz = pi.z - (sj.z + shz);
-              const double rsq = delx * delx + dely * dely + delz * delz;
-              const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
-              double p;
-              const int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
-              const double4 c = lds64[m];
-              acc += okf * (((c.x * p + c.y) * p + c.z) * p + c.w);
-            });
-      }
-    }
-    acc = row_sum<TPR>(acc);
-    if (lane == 0 && r < a.R) rho_out[a.row_point[r]] = acc;
-  }
+  for (long ch = s.first; ch < s.end; ch += s.stride)
+    density_chunk<TPR, MIXED, NT, U>(a, ch, offs, tab32, lds64, rdr, nr, rho_out);
 }
 
 // ------------------------------------------------------------------ embedding derivative per site
@@ -489,13 +498,13 @@ __device__ __forceinline__ double fprime(const double* __restrict__ frho, double
 // densities for an interpolated atom (pair_eam.cpp:700-760); two live sites
 // per thread in dense order.  Only the F' word of each record is written.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, UcellGrid ug,
-                                               const int* __restrict__ lsdesc, const int* __restrict__ nlive, int A,
-                                               const double* __restrict__ frho, double rdrho, int nrho,
-                                               double4* __restrict__ site_live, float4* __restrict__ site32) {
+__device__ __forceinline__ void embed_block(int blk, const double* __restrict__ rho, const UcellGrid& ug,
+                                            const int* __restrict__ lsdesc, const int* __restrict__ nlive, int A,
+                                            const double* __restrict__ frho, double rdrho, int nrho,
+                                            double4* __restrict__ site_live, float4* __restrict__ site32) {
 #pragma clang fp contract(off)
   const int n = *nlive;
-  const int k0 = blockIdx.x * blockDim.x * RSPT + threadIdx.x;
+  const int k0 = blk * blockDim.x * RSPT + threadIdx.x;
   int kk[RSPT], d[RSPT];
 #pragma unroll
   for (int t = 0; t < RSPT; t++) {
@@ -525,6 +534,14 @@ __global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, U
   }
 }
 
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_embed(const double* __restrict__ rho, UcellGrid ug,
+                                               const int* __restrict__ lsdesc, const int* __restrict__ nlive, int A,
+                                               const double* __restrict__ frho, double rdrho, int nrho,
+                                               double4* __restrict__ site_live, float4* __restrict__ site32) {
+  embed_block<MIXED>(blockIdx.x, rho, ug, lsdesc, nlive, A, frho, rdrho, nrho, site_live, site32);
+}
+
 // ------------------------------------------------------------------ EAM force (pair_eam.cpp:853-1027)
 
 struct EamForceArgs {
@@ -535,6 +552,154 @@ struct EamForceArgs {
   int nr, nrho;
   double escale_node, vscale_node;
 };
+
+// Force (and, with EV, energy and virial) of the rows of chunk ch of one
+// model; as density_chunk.
+template <int TPR, bool MIXED, bool EV, int NT, int U>
+__device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArgs& q, long ch, const float4* offs,
+                                            const float4* tab32, const double4* lds64, double* __restrict__ f,
+                                            double* __restrict__ gf, int P, int E8, double* __restrict__ erow) {
+  const int lane = threadIdx.x % TPR;
+  const int rsub = threadIdx.x / TPR;
+  const float rdr32 = (float)q.rdr;
+  const int r = (int)(ch * (NT / TPR)) + rsub;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  double ephi = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+  int k = 0;
+  if (r < a.R) {
+    k = a.row_site[r];
+    const int n = a.nnum[r];
+    const unsigned row0 = (unsigned)r * (unsigned)a.cap;
+    if (MIXED) {
+      const float4 pi = a.site32[k];
+      const float cut32 = (float)a.cutsq;
+      walk_row<TPR, U, float4>(
+          a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
+          [&](unsigned ent, const float4& sj, bool valid) {
+            const float4 o = offs[ent >> MIX_SHIFT];
+            const float dx = (pi.x - sj.x) - o.x;
+            const float dy = (pi.y - sj.y) - o.y;
+            const float dz = (pi.z - sj.z) - o.z;
+            // rsq is floored so that a masked entry (the row's own site past
+            // the row's end) stays finite and contributes 0 * finite
+            const float rsq = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+            const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
+            const float recip = __builtin_amdgcn_rsqf(rsq);
+            const float rr = rsq * recip;
+            float p;
+            const int m = knot_r32(rr, rdr32, q.nr, p);
+            const float4 cz = tab32[2 * m];      // z2r value c3..c6
+            const float4 cd = tab32[2 * m + 1];  // rhor derivative c0..c2, z2r c2
+            const float rhop = (cd.x * p + cd.y) * p + cd.z;
+            // z2 = ((c3 p + c4) p + c5) p + c6 by Horner, and dz2/dp from
+            // the same partial sums; z2' = dz2/dp * rdr
+            const float h1 = cz.x * p + cz.y;
+            const float h2 = h1 * p + cz.z;
+            const float z2 = h2 * p + cz.w;
+            const float dz2 = (cz.x * p + h1) * p + h2;
+            const float phi = z2 * recip;
+            const float phip = (dz2 * rdr32 - phi) * recip;
+            const float fpair = okf * (-((pi.w + sj.w) * rhop + phip) * recip);
+            const double fp = (double)fpair;
+            fx = __builtin_fma((double)dx, fp, fx);
+            fy = __builtin_fma((double)dy, fp, fy);
+            fz = __builtin_fma((double)dz, fp, fz);
+            if (EV) {
+              ephi += (double)(okf * phi);
+              v0 += (double)(dx * dx * fpair);
+              v1 += (double)(dy * dy * fpair);
+              v2 += (double)(dz * dz * fpair);
+              v3 += (double)(dx * dy * fpair);
+              v4 += (double)(dx * dz * fpair);
+              v5 += (double)(dy * dz * fpair);
+            }
+          });
+    } else {
+      const double4 pi = a.site[k];
+      walk_row<TPR, U, double4>(
+          a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
+          [&](unsigned ent, const double4& sj, bool valid) {
+            double shx, shy, shz;
+            img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
+            const double delx = pi.x - (sj.x + shx);
+            const double dely = pi.y - (sj.y + shy);
+            const double delz = pi.z - (sj.z + shz);
+            const double rsq = fmax(delx * delx + dely * dely + delz * delz, 1e-200);
+            const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
+            const double rr = sqrt(rsq);
+            double p;
+            const int m = knot_r<double>(rr, q.rdr, q.nr, p);
+            const double4 cr = lds64[2 * m];
+            const double4 cz = lds64[2 * m + 1];
+            const double rhop = ((3.0 * cr.x * p + 2.0 * cr.y) * p + cr.z) * q.rdr;
+            const double z2p = ((3.0 * cz.x * p + 2.0 * cz.y) * p + cz.z) * q.rdr;
+            const double z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
+            const double recip = 1.0 / rr;
+            const double phi = z2 * recip;
+            const double phip = z2p * recip - phi * recip;
+            const double psip = pi.w * rhop + sj.w * rhop + phip;
+            const double fpair = okf * (-1.0 * psip * recip);
+            fx += delx * fpair;
+            fy += dely * fpair;
+            fz += delz * fpair;
+            if (EV) {
+              ephi += okf * phi;
+              v0 += delx * delx * fpair;
+              v1 += dely * dely * fpair;
+              v2 += delz * delz * fpair;
+              v3 += delx * dely * fpair;
+              v4 += delx * delz * fpair;
+              v5 += dely * delz * fpair;
+            }
+          });
+    }
+  }
+  fx = row_sum<TPR>(fx);
+  fy = row_sum<TPR>(fy);
+  fz = row_sum<TPR>(fz);
+  if (EV) {
+    ephi = row_sum<TPR>(ephi);
+    v0 = row_sum<TPR>(v0);
+    v1 = row_sum<TPR>(v1);
+    v2 = row_sum<TPR>(v2);
+    v3 = row_sum<TPR>(v3);
+    v4 = row_sum<TPR>(v4);
+    v5 = row_sum<TPR>(v5);
+  }
+  if (lane == 0 && r < a.R) {
+    const int pnt = a.row_point[r];
+    const bool is_atom = pnt < a.A;
+    if (is_atom) {
+      f[pnt] = fx;
+      f[(size_t)P + pnt] = fy;
+      f[2 * (size_t)P + pnt] = fz;
+    } else {
+      const int g = pnt - a.A;
+      gf[g] = fx;
+      gf[(size_t)E8 + g] = fy;
+      gf[2 * (size_t)E8 + g] = fz;
+    }
+    if (EV) {
+      // embedding energy F(rho) at atoms (weight 1) and nodes (weight nodal_weight)
+      const double rho = q.rho[pnt];
+      const double fpi = fprime(q.frho, rho, q.rdrho, q.nrho);
+      double p;
+      const int m = knot_rho(rho, q.rdrho, q.nrho, p);
+      const double* c = q.frho + (size_t)m * 7;
+      double phi = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+      if (rho > q.rhomax) phi += fpi * (rho - q.rhomax);
+      const double es = is_atom ? 0.5 : q.escale_node;
+      const double vs = is_atom ? 0.5 : q.vscale_node;
+      erow[r] = phi * (is_atom ? 1.0 : 2.0 * q.escale_node) + es * ephi;
+      erow[(size_t)a.R * 1 + r] = vs * v0;
+      erow[(size_t)a.R * 2 + r] = vs * v1;
+      erow[(size_t)a.R * 3 + r] = vs * v2;
+      erow[(size_t)a.R * 4 + r] = vs * v3;
+      erow[(size_t)a.R * 5 + r] = vs * v4;
+      erow[(size_t)a.R * 6 + r] = vs * v5;
+    }
+  }
+}
 
 template <int TPR, bool MIXED, bool EV, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, double* __restrict__ f,
@@ -551,149 +716,98 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
   } else {
     stage_table(lds64, static_cast<const double4*>(q.tab), 2 * (q.nr + 1));
   }
-  const int lane = threadIdx.x % TPR;
-  const int rsub = threadIdx.x / TPR;
   const Sched s = sched(a.nchunks);
-  const float rdr32 = (float)q.rdr;
+  for (long ch = s.first; ch < s.end; ch += s.stride)
+    force_chunk<TPR, MIXED, EV, NT, U>(a, q, ch, offs, tab32, lds64, f, gf, P, E8, erow);
+}
+
+// ------------------------------------------------------------------ several models in one launch
+// The density, embedding and force passes of up to MAXSEG mixed-precision EAM
+// models that share the potential and the launch shape, one launch each (the
+// serving loop's replicas).  The models' chunks of rows (blocks of live sites
+// for the embedding) are laid end to end; a workgroup finds the model of each
+// chunk from the running counts and, when the model changes, stages that
+// model's cell-offset vectors.  A row is walked by the same lanes in the same
+// order as in the model's own launch, so its sums are bitwise those of a lone
+// run.
+
+constexpr int MAXSEG = 16;
+
+struct SegRows {
+  PairArgs a[MAXSEG];
+  double* out[MAXSEG];  // density: rho; force: f
+  double* gf[MAXSEG];
+  int P[MAXSEG], E8[MAXSEG];
+  long first[MAXSEG + 1];  // first chunk of each model; first[n] = all chunks
+  int n;
+};
+
+struct SegSites {
+  const double* rho[MAXSEG];
+  UcellGrid ug[MAXSEG];
+  const int* lsdesc[MAXSEG];
+  const int* nlive[MAXSEG];
+  float4* site32[MAXSEG];
+  int A[MAXSEG];
+  int first[MAXSEG + 1];  // first block of each model
+  int n;
+};
+
+// the model of chunk ch (the same for every lane of the workgroup)
+__device__ __forceinline__ int seg_of(const long* first, int n, long ch) {
+  int k = 0;
+  while (k + 1 < n && ch >= first[k + 1]) k++;
+  return k;
+}
+
+template <int TPR, int NT, int U>
+__global__ __launch_bounds__(NT) void k_eam_density_seg(const SegRows sg, const float4* __restrict__ tab, double rdr,
+                                                        int nr) {
+  __shared__ float4 offs[NOFF_PAD];
+  extern __shared__ float4 lds_raw[];
+  float4* tab32 = lds_raw;
+  stage_table(tab32, tab, nr + 1);
+  const Sched s = sched(sg.first[sg.n]);
+  int cur = -1;
   for (long ch = s.first; ch < s.end; ch += s.stride) {
-    const int r = (int)(ch * (NT / TPR)) + rsub;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    double ephi = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
-    int k = 0;
-    if (r < a.R) {
-      k = a.row_site[r];
-      const int n = a.nnum[r];
-      const unsigned row0 = (unsigned)r * (unsigned)a.cap;
-      if (MIXED) {
-        const float4 pi = a.site32[k];
-        const float cut32 = (float)a.cutsq;
-        walk_row<TPR, U, float4>(
-            a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
-            [&](unsigned ent, const float4& sj, bool valid) {
-              const float4 o = offs[ent >> MIX_SHIFT];
-              const float dx = (pi.x - sj.x) - o.x;
-              const float dy = (pi.y - sj.y) - o.y;
-              const float dz = (pi.z - sj.z) - o.z;
-              // rsq is floored so that a masked entry (the row's own site past
-              // the row's end) stays finite and contributes 0 * finite
-              const float rsq = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
-              const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
-              const float recip = __builtin_amdgcn_rsqf(rsq);
-              const float rr = rsq * recip;
-              float p;
-              const int m = knot_r32(rr, rdr32, q.nr, p);
-              const float4 cz = tab32[2 * m];      // z2r value c3..c6
-              const float4 cd = tab32[2 * m + 1];  // rhor derivative c0..c2, z2r c2
-              const float rhop = (cd.x * p + cd.y) * p + cd.z;
-              // z2 = ((c3 p + c4) p + c5) p + c6 by Horner, and dz2/dp from
-              // the same partial sums; z2' = dz2/dp * rdr
-              const float h1 = cz.x * p + cz.y;
-              const float h2 = h1 * p + cz.z;
-              const float z2 = h2 * p + cz.w;
-              const float dz2 = (cz.x * p + h1) * p + h2;
-              const float phi = z2 * recip;
-              const float phip = (dz2 * rdr32 - phi) * recip;
-              const float fpair = okf * (-((pi.w + sj.w) * rhop + phip) * recip);
-              const double fp = (double)fpair;
-              fx = __builtin_fma((double)dx, fp, fx);
-              fy = __builtin_fma((double)dy, fp, fy);
-              fz = __builtin_fma((double)dz, fp, fz);
-              if (EV) {
-                ephi += (double)(okf * phi);
-                v0 += (double)(dx * dx * fpair);
-                v1 += (double)(dy * dy * fpair);
-                v2 += (double)(dz * dz * fpair);
-                v3 += (double)(dx * dy * fpair);
-                v4 += (double)(dx * dz * fpair);
-                v5 += (double)(dy * dz * fpair);
-              }
-            });
-      } else {
-        const double4 pi = a.site[k];
-        walk_row<TPR, U, double4>(
-            a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
-            [&](unsigned ent, const double4& sj, bool valid) {
-              double shx, shy, shz;
-              img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
-              const double delx = pi.x - (sj.x + shx);
-              const double dely = pi.y - (sj.y + shy);
-              const double delz = pi.z - (sj.z + shz);
-              const double rsq = fmax(delx * delx + dely * dely + delz * delz, 1e-200);
-              const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
-              const double rr = sqrt(rsq);
-              double p;
-              const int m = knot_r<double>(rr, q.rdr, q.nr, p);
-              const double4 cr = lds64[2 * m];
-              const double4 cz = lds64[2 * m + 1];
-              const double rhop = ((3.0 * cr.x * p + 2.0 * cr.y) * p + cr.z) * q.rdr;
-              const double z2p = ((3.0 * cz.x * p + 2.0 * cz.y) * p + cz.z) * q.rdr;
-              const double z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
-              const double recip = 1.0 / rr;
-              const double phi = z2 * recip;
-              const double phip = z2p * recip - phi * recip;
-              const double psip = pi.w * rhop + sj.w * rhop + phip;
-              const double fpair = okf * (-1.0 * psip * recip);
-              fx += delx * fpair;
-              fy += dely * fpair;
-              fz += delz * fpair;
-              if (EV) {
-                ephi += okf * phi;
-                v0 += delx * delx * fpair;
-                v1 += dely * dely * fpair;
-                v2 += delz * delz * fpair;
-                v3 += delx * dely * fpair;
-                v4 += delx * delz * fpair;
-                v5 += dely * delz * fpair;
-              }
-            });
-      }
+    const int k = seg_of(sg.first, sg.n, ch);
+    if (k != cur) {
+      __syncthreads();  // every lane is done with the previous model's offsets
+      stage_offsets(offs, sg.a[k].cs);
+      __syncthreads();
+      cur = k;
     }
-    fx = row_sum<TPR>(fx);
-    fy = row_sum<TPR>(fy);
-    fz = row_sum<TPR>(fz);
-    if (EV) {
-      ephi = row_sum<TPR>(ephi);
-      v0 = row_sum<TPR>(v0);
-      v1 = row_sum<TPR>(v1);
-      v2 = row_sum<TPR>(v2);
-      v3 = row_sum<TPR>(v3);
-      v4 = row_sum<TPR>(v4);
-      v5 = row_sum<TPR>(v5);
-    }
-    if (lane == 0 && r < a.R) {
-      const int pnt = a.row_point[r];
-      const bool is_atom = pnt < a.A;
-      if (is_atom) {
-        f[pnt] = fx;
-        f[(size_t)P + pnt] = fy;
-        f[2 * (size_t)P + pnt] = fz;
-      } else {
-        const int g = pnt - a.A;
-        gf[g] = fx;
-        gf[(size_t)E8 + g] = fy;
-        gf[2 * (size_t)E8 + g] = fz;
-      }
-      if (EV) {
-        // embedding energy F(rho) at atoms (weight 1) and nodes (weight nodal_weight)
-        const double rho = q.rho[pnt];
-        const double fpi = fprime(q.frho, rho, q.rdrho, q.nrho);
-        double p;
-        const int m = knot_rho(rho, q.rdrho, q.nrho, p);
-        const double* c = q.frho + (size_t)m * 7;
-        double phi = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
-        if (rho > q.rhomax) phi += fpi * (rho - q.rhomax);
-        const double es = is_atom ? 0.5 : q.escale_node;
-        const double vs = is_atom ? 0.5 : q.vscale_node;
-        erow[r] = phi * (is_atom ? 1.0 : 2.0 * q.escale_node) + es * ephi;
-        erow[(size_t)a.R * 1 + r] = vs * v0;
-        erow[(size_t)a.R * 2 + r] = vs * v1;
-        erow[(size_t)a.R * 3 + r] = vs * v2;
-        erow[(size_t)a.R * 4 + r] = vs * v3;
-        erow[(size_t)a.R * 5 + r] = vs * v4;
-        erow[(size_t)a.R * 6 + r] = vs * v5;
-      }
-    }
+    density_chunk<TPR, true, NT, U>(sg.a[k], ch - sg.first[k], offs, tab32, nullptr, rdr, nr, sg.out[k]);
   }
+}
+
+template <int TPR, int NT, int U>
+__global__ __launch_bounds__(NT) void k_eam_force_seg(const SegRows sg, EamForceArgs q) {
+  __shared__ float4 offs[NOFF_PAD];
+  extern __shared__ float4 lds_raw[];
+  float4* tab32 = lds_raw;
+  stage_table(tab32, static_cast<const float4*>(q.tab), 2 * (q.nr + 1));
+  const Sched s = sched(sg.first[sg.n]);
+  int cur = -1;
+  for (long ch = s.first; ch < s.end; ch += s.stride) {
+    const int k = seg_of(sg.first, sg.n, ch);
+    if (k != cur) {
+      __syncthreads();
+      stage_offsets(offs, sg.a[k].cs);
+      __syncthreads();
+      cur = k;
+    }
+    force_chunk<TPR, true, false, NT, U>(sg.a[k], q, ch - sg.first[k], offs, tab32, nullptr, sg.out[k], sg.gf[k],
+                                         sg.P[k], sg.E8[k], nullptr);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_embed_seg(const SegSites sg, const double* __restrict__ frho, double rdrho,
+                                                   int nrho) {
+  int k = 0;
+  while (k + 1 < sg.n && (int)blockIdx.x >= sg.first[k + 1]) k++;
+  embed_block<true>((int)blockIdx.x - sg.first[k], sg.rho[k], sg.ug[k], sg.lsdesc[k], sg.nlive[k], sg.A[k], frho,
+                    rdrho, nrho, nullptr, sg.site32[k]);
 }
 
 // ------------------------------------------------------------------ LJ (pair_lj_cut.cpp:278-414)
@@ -987,14 +1101,81 @@ static void launch_pair(Ctx& c, int eflag, int vflag, Gate g) {
   }
 }
 
-void pair_passes(Ctx& c, int eflag, int vflag, Gate g) {
-  const bool ev = eflag || vflag;
-  if (ev) c.erow.reserve((size_t)7 * c.P);
+void ensure_inner(Ctx& c) {
   if (c.inner_mode != c.precision) {  // precision or delta changed since the last prune
     site_refresh(c, true);
     build_inner(c);
   }
+}
+
+void pair_passes(Ctx& c, int eflag, int vflag, Gate g) {
+  const bool ev = eflag || vflag;
+  if (ev) c.erow.reserve((size_t)7 * c.P);
+  ensure_inner(c);
   launch_pair(c, eflag, vflag, g);
+}
+
+bool segmentable(const Ctx& c) {
+  return c.precision == 1 && c.pairstyle == 2 && c.tpr == 8 && c.tpr_force == 4 && c.dens_nt == 512 &&
+         c.pair_shape == 1 && !c.timing && c.P > 0;
+}
+
+int max_segments() { return MAXSEG; }
+
+void pair_passes_segmented(Ctx* const* cs, int n, hipStream_t stream) {
+  if (n < 1 || n > MAXSEG) throw HipError{hipErrorInvalidValue, "segmented pair passes: 1 to 16 models"};
+  Ctx& c0 = *cs[0];
+  SegRows dr{}, fr{};
+  SegSites es{};
+  long cd = 0, cf = 0;
+  int cb = 0;
+  for (int k = 0; k < n; k++) {
+    Ctx& c = *cs[k];
+    // the first model's tables serve all: the same potential and cutoff
+    if (!segmentable(c) || c.nr != c0.nr || c.rdr != c0.rdr || c.nrho != c0.nrho || c.rdrho != c0.rdrho ||
+        c.cutforcesq != c0.cutforcesq || c.inner_mode != c.precision)
+      throw HipError{hipErrorInvalidValue, "segmented pair passes: models differ in potential or set-up"};
+    dr.a[k] = pair_args(c, c.cutforcesq, 512 / 8);
+    dr.first[k] = cd;
+    cd += dr.a[k].nchunks;
+    dr.out[k] = c.rho.get();
+    fr.a[k] = pair_args(c, c.cutforcesq, 1024 / 4);
+    fr.first[k] = cf;
+    cf += fr.a[k].nchunks;
+    fr.out[k] = c.f.get();
+    fr.gf[k] = c.gf.get();
+    fr.P[k] = c.P;
+    fr.E8[k] = NPE * c.E;
+    es.rho[k] = c.rho.get();
+    es.ug[k] = ucell_grid(c);
+    es.lsdesc[k] = c.lsdesc.get();
+    es.nlive[k] = c.d_nlive.get();
+    es.site32[k] = c.site32.get();
+    es.A[k] = c.A;
+    es.first[k] = cb;
+    cb += grid_for(c.S, 256 * RSPT);
+  }
+  dr.first[n] = cd;
+  fr.first[n] = cf;
+  es.first[n] = cb;
+  dr.n = fr.n = es.n = n;
+  const size_t lds_rho = sizeof(float4) * (c0.nr + 1);
+  auto kd = k_eam_density_seg<8, 512, 4>;
+  kd<<<resident_grid(c0, kd, 512, lds_rho, cd), 512, lds_rho, stream>>>(
+      dr, reinterpret_cast<const float4*>(c0.tab32r.get()), c0.rdr, c0.nr);
+  k_embed_seg<<<cb, 256, 0, stream>>>(es, c0.frho_spl.get(), c0.rdrho, c0.nrho);
+  EamForceArgs q{};
+  q.tab = c0.tab32f.get();
+  q.frho = c0.frho_spl.get();
+  q.rdr = c0.rdr;
+  q.rdrho = c0.rdrho;
+  q.rhomax = c0.rhomax;
+  q.nr = c0.nr;
+  q.nrho = c0.nrho;
+  const size_t lds_force = 2 * sizeof(float4) * (c0.nr + 1);
+  auto kf = k_eam_force_seg<4, 1024, 4>;
+  kf<<<resident_grid(c0, kf, 1024, lds_force, cf), 1024, lds_force, stream>>>(fr, q);
+  CAC_HIP(hipGetLastError());
 }
 
 void pair_sums(Ctx& c, int eflag, int vflag) {

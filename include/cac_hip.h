@@ -212,6 +212,17 @@ int cac_hip_server_state(cac_hip_server *srv, long id, double *x, double *nodex)
  * any replica was active; peak slots in use; peak HBM the slots held */
 int cac_hip_server_stats(cac_hip_server *srv, double *p50_ms, double *p99_ms, long *nsteps, double *atom_steps_per_s,
                          int *peak_active, double *peak_hbm_bytes);
+/* Segmented pair passes so far: groups launched (one density, one embedding
+ * and one force launch each, over 2 to 16 replicas) and the replica steps they
+ * covered.  With CAC_HIP_SERVE_SEGMENT=1 set when the server is created, a
+ * stepping thread runs the pair passes of its mixed-precision EAM replicas of
+ * at most CAC_HIP_SERVE_SEGMENT_MAXP points this way; energy steps and other
+ * precisions or pair styles run alone.  Off by default: the replicas' own
+ * streams overlap on the device, and a group's shared launches make each of
+ * its replicas wait for all of them.  Results are bitwise those of a lone run
+ * either way.  No counterpart in the reference, which runs one model per
+ * process. */
+int cac_hip_server_segments(cac_hip_server *srv, long *launches, long *replica_steps);
 
 #ifdef __cplusplus
 }

@@ -44,12 +44,14 @@ def _specs(n, seed):
 
 
 @pytest.mark.gpu
-def test_replicas_same_alone_and_in_the_mix():
+def test_replicas_same_alone_and_in_the_mix(monkeypatch):
     """Six replicas of uneven size, temperature and strain, 150 mixed steps
     each: run alone through the resident loop, then together on three slots
     (so slots are reused by replicas of other sizes, and replicas overlap on
-    the device), then one at a time under a budget that admits a single
-    replica: final positions bitwise equal, final energies equal."""
+    the device), then on four slots stepped by one thread with segmented pair
+    passes switched on (one launch per pass over the thread's replicas), then one at a time
+    under a budget that admits a single replica: final positions bitwise
+    equal, final energies equal."""
     import cac_amd
     specs = _specs(6, 11)
     dks = [decks.serving_deck(s) for s in specs]
@@ -60,7 +62,13 @@ def test_replicas_same_alone_and_in_the_mix():
         th = m.run(NSTEP, 0.001)
         alone.append((m.atoms()[0], m.elements()["nodex"] if d.nelements else None, th[-1]))
         m.close()
-    for slots, budget in ((3, 0.0), (2, 1.0)):
+    for slots, budget, steppers in ((3, 0.0, None), (4, 0.0, "1"), (2, 1.0, None)):
+        if steppers:
+            monkeypatch.setenv("CAC_HIP_SERVE_STEPPERS", steppers)
+            monkeypatch.setenv("CAC_HIP_SERVE_SEGMENT", "1")
+        else:
+            monkeypatch.delenv("CAC_HIP_SERVE_STEPPERS", raising=False)
+            monkeypatch.delenv("CAC_HIP_SERVE_SEGMENT", raising=False)
         srv = cac_amd.Server(slots=slots, hbm_budget_bytes=budget, eam_tables=TAB)
         ids = [srv.submit(d, NSTEP, 0.001, skin=2.0, keep_state=True) for d in dks]
         srv.drain()
@@ -70,6 +78,11 @@ def test_replicas_same_alone_and_in_the_mix():
             assert 2 <= st["peak_active"] <= slots, st  # replicas overlapped
         else:
             assert st["peak_active"] == 1, st
+        if not steppers:
+            assert st["segmented_launches"] == 0, st  # off unless asked for
+        if steppers == "1":
+            # one thread steps up to four replicas: their pair passes were segmented
+            assert st["segmented_launches"] > 0 and st["segmented_steps"] >= 2 * st["segmented_launches"], st
         for rid, d, (x0, nx0, th0) in zip(ids, dks, alone):
             r = srv.result(rid)
             assert r["status"] == 0 and r["nsteps"] == NSTEP
