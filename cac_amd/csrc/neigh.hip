@@ -18,64 +18,6 @@
 
 namespace cac {
 
-// ------------------------------------------------------------------ scans
-constexpr int SCAN_BLOCK = 1024;
-
-__global__ void k_scan_block(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n) {
-  __shared__ int s[SCAN_BLOCK];
-  const int t = threadIdx.x;
-  const long i = (long)blockIdx.x * SCAN_BLOCK + t;
-  const int v = (i < n) ? in[i] : 0;
-  s[t] = v;
-  __syncthreads();
-  for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
-    int a = (t >= off) ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += a;
-    __syncthreads();
-  }
-  if (i < n) out[i] = s[t] - v;
-  if (t == SCAN_BLOCK - 1) bsum[blockIdx.x] = s[t];
-}
-
-__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ boff, int n) {
-  const long i = (long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
-  if (i < n) out[i] += boff[blockIdx.x];
-}
-
-// Exclusive scan of n ints; *total (host) receives the sum.  Deterministic.
-void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total) {
-  if (n <= 0) {
-    if (total) *total = 0;
-    return;
-  }
-  int nb1 = grid_for(n, SCAN_BLOCK);
-  int nb2 = grid_for(nb1, SCAN_BLOCK);
-  int nb3 = grid_for(nb2, SCAN_BLOCK);
-  if (nb3 > 1) throw HipError{hipErrorInvalidValue, "scan longer than 2^30"};
-  c.scan_tmp.reserve((size_t)2 * nb1 + 2 * nb2 + 4);
-  int* b1 = c.scan_tmp.get();
-  int* o1 = b1 + nb1;
-  int* b2 = o1 + nb1;
-  int* o2 = b2 + nb2;
-  int* b3 = o2 + nb2;
-  k_scan_block<<<nb1, SCAN_BLOCK, 0, c.stream>>>(in, out, b1, n);
-  if (nb1 > 1) {
-    k_scan_block<<<nb2, SCAN_BLOCK, 0, c.stream>>>(b1, o1, b2, nb1);
-    if (nb2 > 1) {
-      k_scan_block<<<1, SCAN_BLOCK, 0, c.stream>>>(b2, o2, b3, nb2);
-      k_scan_add<<<nb2, SCAN_BLOCK, 0, c.stream>>>(o1, o2, nb1);
-    }
-    k_scan_add<<<nb1, SCAN_BLOCK, 0, c.stream>>>(out, o1, n);
-  }
-  if (total) {
-    int last_in = 0, last_out = 0;
-    CAC_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CAC_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CAC_HIP(hipStreamSynchronize(c.stream));
-    *total = last_in + last_out;
-  }
-}
 
 // ------------------------------------------------------------------ pbc
 

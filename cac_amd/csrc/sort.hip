@@ -6,11 +6,34 @@
 // cell's sites in candidate order, so the layout is deterministic, and moves
 // the descriptors with the keys (no gather through a permutation afterwards);
 // cell_start[c] is the first slot of cell c (cell_start[ncells] = S).
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "cac_ctx.hpp"
 
 namespace cac {
+
+// Exclusive scan of n ints (single-pass, decoupled look-back); *total (host)
+// receives the sum.  Integer sums are exact in any order, so the result is
+// deterministic.
+void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total) {
+  if (n <= 0) {
+    if (total) *total = 0;
+    return;
+  }
+  size_t tmp = 0;
+  CAC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, c.stream));
+  c.scan_tmp.reserve(tmp / sizeof(int) + 16);
+  CAC_HIP(hipcub::DeviceScan::ExclusiveSum(c.scan_tmp.get(), tmp, in, out, n, c.stream));
+  if (total) {
+    int last_in = 0, last_out = 0;
+    CAC_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CAC_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CAC_HIP(hipStreamSynchronize(c.stream));
+    *total = last_in + last_out;
+  }
+}
 
 // slot i opens the cells after the previous slot's cell up to its own
 __global__ void k_cell_bounds(const int* __restrict__ key, int S, long ncells, int* __restrict__ cell_start) {
@@ -76,10 +99,14 @@ void order_rows(Ctx& c) {
   unsigned long long* key = c.row_key.get();
   int* idx = c.row_idx.get();
   k_row_keys<<<grid_for(R, 256), 256, 0, c.stream>>>(c.row_site.get(), c.scell.get(), R, key, idx);
+  // the Morton key holds 3 bits per bit of the largest cell index
+  int b = 1;
+  while (b < 21 && (1L << b) < std::max(c.nc[0], std::max(c.nc[1], c.nc[2]))) b++;
+  const int bits = 3 * b;
   size_t tmp = 0;
-  CAC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key + R, idx, idx + R, R, 0, 64, c.stream));
+  CAC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key + R, idx, idx + R, R, 0, bits, c.stream));
   c.sort_tmp.reserve(tmp + 16);
-  CAC_HIP(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp.get(), tmp, key, key + R, idx, idx + R, R, 0, 64,
+  CAC_HIP(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp.get(), tmp, key, key + R, idx, idx + R, R, 0, bits,
                                              c.stream));
   c.row_tmp.reserve(2 * (size_t)R);
   int* rs = c.row_tmp.get();
