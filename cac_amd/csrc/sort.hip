@@ -9,6 +9,7 @@
 #include <algorithm>
 
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "cac_ctx.hpp"
 
@@ -103,11 +104,16 @@ void order_rows(Ctx& c) {
   int b = 1;
   while (b < 21 && (1L << b) < std::max(c.nc[0], std::max(c.nc[1], c.nc[2]))) b++;
   const int bits = 3 * b;
+  // onesweep LSD passes over the key's bits at every size: below a million
+  // items the library would otherwise merge-sort, in about 20 launches
+  using OnesweepOnly = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                  rocprim::default_config, 0>;
   size_t tmp = 0;
-  CAC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key + R, idx, idx + R, R, 0, bits, c.stream));
+  CAC_HIP(rocprim::radix_sort_pairs<OnesweepOnly>(nullptr, tmp, key, key + R, idx, idx + R, (unsigned)R, 0u,
+                                                  (unsigned)bits, c.stream));
   c.sort_tmp.reserve(tmp + 16);
-  CAC_HIP(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp.get(), tmp, key, key + R, idx, idx + R, R, 0, bits,
-                                             c.stream));
+  CAC_HIP(rocprim::radix_sort_pairs<OnesweepOnly>(c.sort_tmp.get(), tmp, key, key + R, idx, idx + R, (unsigned)R,
+                                                  0u, (unsigned)bits, c.stream));
   c.row_tmp.reserve(2 * (size_t)R);
   int* rs = c.row_tmp.get();
   int* rp = rs + R;
