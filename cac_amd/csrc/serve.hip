@@ -1,9 +1,10 @@
 // serve.hip -- the resident serving loop of include/cac_hip.h (cac_hip_server_*).
 //
 // Independent replicas (whole models, each a Verlet::run of its own,
-// verlet.cpp:218-360) are submitted from any host thread and served by two
+// verlet.cpp:218-360) are submitted from any host thread and served by
 // service threads over a fixed set of slots: a loader that admits and loads
-// replicas, and a stepper that steps the loaded ones.  A slot is a device context
+// replicas, stepping threads that step the loaded ones, and a rebuilder that
+// finishes the steps that need an outer-list rebuild.  A slot is a device context
 // with its own stream; it keeps its buffers from one replica to the next, so
 // serving never frees device memory (a free synchronizes the whole device).
 //
@@ -70,6 +71,7 @@ struct Active {
   int done = 0;
   double t_admit = 0.0, t_first = 0.0, t_last = 0.0;
   bool issued = false;
+  bool away = false;  // handed to the rebuilder for this step
 };
 
 // Device bytes a model needs, from its sizes: per point the two lists
@@ -105,6 +107,13 @@ struct cac_hip_server {
   bool stop = false, loader_done = false;
   std::vector<std::thread> steppers;
   std::thread loader;
+  // replicas whose step needs an outer-list rebuild, taken off their stepping
+  // thread (a rebuild waits on the host for counts) and stepped by the rebuilder
+  std::deque<Active> rebuild_q;
+  std::condition_variable cv_rebuild;
+  bool rebuild_stop = false;
+  std::thread rebuilder;
+  long rebuilds_away = 0;
   Clock::time_point t0;
 
   // statistics, written by the service thread under mu
@@ -281,6 +290,50 @@ void load_loop(cac_hip_server* s) {
   }
 }
 
+// The rebuilder thread: finishes the steps whose check asked for an outer
+// rebuild (rebuild, pair passes, closing kick, as step_resolve does on a
+// stepping thread), then returns each replica to its stepping thread.  A
+// rebuild reads counts back to the host several times; done here, it does not
+// hold up the other replicas of that stepping thread.
+void rebuild_loop(cac_hip_server* s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    Active a;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_rebuild.wait(lk, [&] { return !s->rebuild_q.empty() || s->rebuild_stop; });
+      if (s->rebuild_q.empty()) return;
+      a = std::move(s->rebuild_q.front());
+      s->rebuild_q.pop_front();
+    }
+    Ctx& c = s->slots[a.slot].h->c;
+    const int ev = a.done + 1 == a.req->r.nsteps ? 1 : 0;
+    try {
+      step_resolve(c, a.req->r.dt, ev, a.done + 1 < a.req->r.nsteps);
+    } catch (const HipError& e) {
+      finish(s, a, code_of(e), e.where);
+      continue;
+    }
+    const double t = s->now();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->interval_ms.push_back((float)(1000.0 * (t - a.t_last)));
+      s->rebuilds_away++;
+    }
+    a.t_last = t;
+    a.done++;
+    a.away = false;
+    if (a.done >= a.req->r.nsteps) {
+      finish(s, a, 0, "");
+      continue;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int me = a.stepper;
+    s->ready[me].push_back(std::move(a));
+    s->cv_ready.notify_all();
+  }
+}
+
 // stream and event of a stepping thread's segmented pair passes
 struct SegStream {
   hipStream_t stream = nullptr;
@@ -409,6 +462,11 @@ void serve(cac_hip_server* s, int me) {
       Ctx& c = s->slots[a.slot].h->c;
       const int ev = a.done + 1 == a.req->r.nsteps ? 1 : 0;
       try {
+        CAC_HIP(hipEventSynchronize(c.ev_check));
+        if (c.h_steps[0] == (int)c.ntimestep) {  // an outer rebuild: the rebuilder takes the step
+          a.away = true;
+          continue;
+        }
         if (segment && !ev && c.P <= seg_maxp && segmentable(c)) {
           step_prepare(c);
           grp.push_back(&a);
@@ -428,11 +486,22 @@ void serve(cac_hip_server* s, int me) {
       s->interval_ms.insert(s->interval_ms.end(), round_ms.begin(), round_ms.end());
     }
     for (Active& a : act) {
-      if (a.done < 0) continue;
+      if (a.done < 0 || a.away) continue;
       if (a.done >= a.req->r.nsteps) {
         finish(s, a, 0, "");
         a.done = -1;
       }
+    }
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      bool any = false;
+      for (Active& a : act)
+        if (a.away && a.done >= 0) {
+          s->rebuild_q.push_back(std::move(a));
+          a.done = -1;  // the moved-from entry leaves this thread's list
+          any = true;
+        }
+      if (any) s->cv_rebuild.notify_all();
     }
     act.erase(std::remove_if(act.begin(), act.end(), [](const Active& a) { return a.done < 0; }), act.end());
   }
@@ -472,6 +541,7 @@ int cac_hip_server_create(cac_hip_server** out, int device, int slots, double hb
   s->ready.resize(nstep);
   s->load_of.assign(nstep, 0);
   for (int k = 0; k < nstep; k++) s->steppers.emplace_back(serve, s, k);
+  s->rebuilder = std::thread(rebuild_loop, s);
   s->loader = std::thread(load_loop, s);
   *out = s;
   return CAC_HIP_OK;
@@ -488,6 +558,12 @@ int cac_hip_server_destroy(cac_hip_server* s) {
   if (s->loader.joinable()) s->loader.join();
   for (std::thread& t : s->steppers)
     if (t.joinable()) t.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->rebuild_stop = true;  // the steppers have left, so no replica is away
+  }
+  s->cv_rebuild.notify_all();
+  if (s->rebuilder.joinable()) s->rebuilder.join();
   for (Slot& sl : s->slots) cac_hip_destroy(sl.h);
   delete s;
   return CAC_HIP_OK;

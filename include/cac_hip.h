@@ -149,9 +149,12 @@ int cac_hip_set_inner_skin(cac_hip_ctx *ctx, double delta);
  * the reference, whose one process runs one model; each replica is a whole
  * Verlet::run of its own model, verlet.cpp:218-360).
  *
- * The server owns `slots` device contexts, each with its own stream, and two
- * service threads: a loader (admission, upload, neighbor build, first force)
- * and a stepper.  Submitted replicas wait in arrival order and are admitted
+ * The server owns `slots` device contexts, each with its own stream, and
+ * service threads: a loader (admission, upload, neighbor build, first force),
+ * stepping threads (CAC_HIP_SERVE_STEPPERS, default 4) that share the active
+ * replicas, and a rebuilder that finishes the steps whose check asks for an
+ * outer-list rebuild (a rebuild reads counts back to the host, so it would
+ * hold up a stepping thread's other replicas).  Submitted replicas wait in arrival order and are admitted
  * when a slot is free and the HBM the replica needs (estimated from its
  * atoms, elements and unit cells) fits the budget beside what the slots
  * hold.  Admitted replicas are stepped round-robin: each round queues the

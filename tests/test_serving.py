@@ -9,6 +9,7 @@ from cac_amd import decks
 
 TAB = decks.cu_eam_tables()
 NSTEP = 150
+SKIN = 0.5  # small enough that thermal motion rebuilds the outer lists within NSTEP
 
 
 def test_serving_stream_follows_its_distributions():
@@ -57,7 +58,7 @@ def test_replicas_same_alone_and_in_the_mix(monkeypatch):
     dks = [decks.serving_deck(s) for s in specs]
     alone = []
     for d in dks:
-        m = cac_amd.Model(d, "eam", skin=2.0, eam_tables=TAB, precision=cac_amd.MIXED)
+        m = cac_amd.Model(d, "eam", skin=SKIN, eam_tables=TAB, precision=cac_amd.MIXED)
         m.setup()
         th = m.run(NSTEP, 0.001)
         alone.append((m.atoms()[0], m.elements()["nodex"] if d.nelements else None, th[-1]))
@@ -70,7 +71,7 @@ def test_replicas_same_alone_and_in_the_mix(monkeypatch):
             monkeypatch.delenv("CAC_HIP_SERVE_STEPPERS", raising=False)
             monkeypatch.delenv("CAC_HIP_SERVE_SEGMENT", raising=False)
         srv = cac_amd.Server(slots=slots, hbm_budget_bytes=budget, eam_tables=TAB)
-        ids = [srv.submit(d, NSTEP, 0.001, skin=2.0, keep_state=True) for d in dks]
+        ids = [srv.submit(d, NSTEP, 0.001, skin=SKIN, keep_state=True) for d in dks]
         srv.drain()
         st = srv.stats()
         assert st["steps"] == 6 * NSTEP
@@ -83,6 +84,9 @@ def test_replicas_same_alone_and_in_the_mix(monkeypatch):
         if steppers == "1":
             # one thread steps up to four replicas: their pair passes were segmented
             assert st["segmented_launches"] > 0 and st["segmented_steps"] >= 2 * st["segmented_launches"], st
+        # some replicas rebuilt their outer lists on the way (those steps are
+        # finished by the server's rebuilder thread)
+        assert max(srv.result(rid)["builds"] for rid in ids) > 1
         for rid, d, (x0, nx0, th0) in zip(ids, dks, alone):
             r = srv.result(rid)
             assert r["status"] == 0 and r["nsteps"] == NSTEP
