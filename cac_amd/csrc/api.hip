@@ -4,6 +4,7 @@
 // (ElementVecCAC::set_element_types and friends), the EAM spline build
 // (PairEAM::array2spline / interpolate), LJ coefficients (PairLJCut::init_one),
 // neighbor cutoffs (Neighbor::init) and the Verlet step order (verlet.cpp:65-360).
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -23,7 +24,6 @@ Ctx::~Ctx() {
   if (h_flags) (void)hipHostFree(h_flags);
   if (h_steps) (void)hipHostFree(const_cast<int*>(h_steps));
   if (h_stage) (void)hipHostFree(h_stage);
-  if (ev_check) (void)hipEventDestroy(ev_check);
   if (ev_seg) (void)hipEventDestroy(ev_seg);
   for (auto& e : ev_pool)
     if (e) (void)hipEventDestroy(e);
@@ -100,16 +100,38 @@ bool step_issue(Ctx& c, double dt, int ev) {
   }
   const Gate g{c.d_steps.get(), N};
   if (!begun) step_begin(c, dt, N, due);
-  CAC_HIP(hipEventRecord(c.ev_check, c.stream));
   site_refresh(c, false, g);
   return true;
+}
+
+// Poll the host-mapped word the check's last workgroup writes.  After 1 ms
+// the stream is also queried (then every 0.1 ms), so a failed launch is an
+// error, not an endless wait; before that the poll touches no runtime call,
+// which would take the runtime's locks.
+void wait_check(Ctx& c) {
+  using clk = std::chrono::steady_clock;
+  const int N = (int)c.ntimestep;
+  volatile int* done = c.h_steps + 2;
+  if (*done == N) return;
+  const auto t0 = clk::now();
+  auto next_query = t0 + std::chrono::milliseconds(1);
+  for (unsigned spin = 1; *done != N; spin++) {
+    if ((spin & 1023u) != 0) continue;
+    const auto t = clk::now();
+    if (t < next_query) continue;
+    next_query = t + std::chrono::microseconds(100);
+    const hipError_t q = hipStreamQuery(c.stream);
+    if (q != hipSuccess && q != hipErrorNotReady) throw HipError{q, "displacement check"};
+    if (q == hipSuccess && *done != N)
+      throw HipError{hipErrorUnknown, "displacement check ended without publishing its verdict"};
+  }
 }
 
 // step_resolve in three parts, so that a caller can run the pair passes of
 // several models together between the first and the last
 void step_prepare(Ctx& c) {
   const int N = (int)c.ntimestep;
-  CAC_HIP(hipEventSynchronize(c.ev_check));
+  wait_check(c);
   if (c.h_steps[0] == N) {
     if (c.ago == std::max(c.every, c.delay)) c.ndanger++;
     build_neighbor(c);
@@ -167,6 +189,7 @@ void reset_model(Ctx& c) {
   CAC_HIP(hipMemsetAsync(c.d_steps.get(), 0xFF, sizeof(int) * 2, c.stream));
   CAC_HIP(hipStreamSynchronize(c.stream));
   c.h_steps[0] = c.h_steps[1] = -1;
+  c.h_steps[2] = -1;  // no check of the new model has published yet
 }
 
 }  // namespace cac
@@ -331,7 +354,9 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     hs[0] = hs[1] = -1;
     c.h_steps = hs;
     CAC_HIP(hipHostGetDevicePointer((void**)&c.h_steps_dev, hs, 0));
-    CAC_HIP(hipEventCreateWithFlags(&c.ev_check, hipEventDisableTiming));
+    c.d_check_count.reserve(1);
+    CAC_HIP(hipMemsetAsync(c.d_check_count.get(), 0, sizeof(unsigned), c.stream));
+    hs[2] = -1;
     CAC_HIP(hipEventCreateWithFlags(&c.ev_seg, hipEventDisableTiming));
     c.units = units;
     if (const char* e = std::getenv("CAC_HIP_TPR")) c.tpr = std::atoi(e);

@@ -282,7 +282,10 @@ struct Ctx {
   DBuf<float4> site32f;           // mixed path: FP32 cell-relative records of every site (for the prune)
   volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
   int* h_steps_dev = nullptr;       // device address of h_steps
-  hipEvent_t ev_check = nullptr;    // recorded after the check of a speculative step
+  // the check's completion: its kernels count their workgroups on this device
+  // counter, and the last one writes the step number into h_steps[2]
+  DBuf<unsigned> d_check_count;
+  unsigned check_blocks = 0;        // workgroups counted so far (the running target)
   hipEvent_t ev_seg = nullptr;      // recorded when the model is ready for segmented pair passes
   double* h_stage = nullptr;        // pinned staging of the per-step host handover
   size_t h_stage_n = 0;
@@ -320,6 +323,7 @@ void step_resolve(Ctx& c, double dt, int ev, bool fuse_next = false);
 // step_resolve = step_prepare (wait for the check, rebuild or re-prune), the
 // pair passes, step_close (nodal forces and the closing kick)
 void step_prepare(Ctx& c);
+void wait_check(Ctx& c);  // until the current step's displacement check has published its verdict
 void step_close(Ctx& c, double dt, int ev, bool fuse_next);
 void reset_model(Ctx& c);  // before loading a new model into a used context
 void compact_live(Ctx& c);  // dense live-site records from the full ones (after a prune)

@@ -150,11 +150,38 @@ void nodal_assembly(Ctx& c) {
 // delta/2 since the last prune).  A wave with a hit writes the step number
 // into the device and host-mapped step words; every writer writes the same
 // value, so plain stores suffice.
-__global__ void k_step_begin(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ f,
+// The check's verdict is complete once every workgroup of the launch has
+// written its step words.  A wave that writes a step word fences at system
+// scope right after (only on a step whose check fires).  Each workgroup then
+// counts itself on a device counter after a barrier; the workgroup that
+// completes the launch's count (the host passes the running total it
+// expects) fences and writes the step number into the host-mapped `done`
+// word, which the host polls instead of waiting on an event recorded in the
+// stream.
+__device__ __forceinline__ void publish_check(unsigned* count, unsigned target, int* done, int step) {
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(count, 1u) == target - 1u) {
+    __threadfence_system();
+    *reinterpret_cast<volatile int*>(done) = step;
+  }
+}
+
+// the step words of a wave whose points hit a trigger
+__device__ __forceinline__ void mark_step(int* d_word, int* h_word, int step) {
+  *d_word = step;
+  *h_word = step;
+  __threadfence_system();
+}
+
+// workgroup size of the fused kick / check kernels: few workgroups, so few
+// counter updates per step
+constexpr int CB = 1024;
+
+__global__ __launch_bounds__(CB) void k_step_begin(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ f,
                              const int* __restrict__ ptype, const double* __restrict__ xh,
                              const double* __restrict__ xhi, int P, MassTab mt, double dtf, double dtv, double trig,
                              double trig_in, int outer, int step, int* __restrict__ d_steps,
-                             int* __restrict__ h_steps) {
+                             int* __restrict__ h_steps, unsigned* __restrict__ count, unsigned target) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   int hit = 0, hit_in = 0;
   if (p < P) {
@@ -175,24 +202,21 @@ __global__ void k_step_begin(double* __restrict__ x, double* __restrict__ v, con
     hit_in = (ex * ex + ey * ey + ez * ez > trig_in);
   }
   const bool lead = (threadIdx.x & 63) == 0;
-  if (__ballot(hit) && lead) {
-    d_steps[0] = step;
-    h_steps[0] = step;
-  }
-  if (__ballot(hit_in) && lead) {
-    d_steps[1] = step;
-    h_steps[1] = step;
-  }
+  if (__ballot(hit) && lead) mark_step(d_steps, h_steps, step);
+  if (__ballot(hit_in) && lead) mark_step(d_steps + 1, h_steps + 1, step);
+  publish_check(count, target, h_steps + 2, step);
 }
 
 void step_begin(Ctx& c, double dt, int step, bool outer) {
   ScopedTimer tm(c, "nve");
   const double dtf = 0.5 * dt * c.ftm2v;
   const double hd = 0.5 * c.inner_skin;
-  k_step_begin<<<grid_for(c.P, 256), 256, 0, c.stream>>>(c.x.get(), c.v.get(), c.f.get(), c.ptype.get(),
-                                                         c.xhold.get(), c.xhold_in.get(), c.P, mass_tab(c), dtf, dt,
-                                                         c.triggersq, hd * hd, outer ? 1 : 0, step, c.d_steps.get(),
-                                                         c.h_steps_dev);
+  const int blocks = grid_for(c.P, CB);
+  c.check_blocks += (unsigned)blocks;
+  k_step_begin<<<blocks, CB, 0, c.stream>>>(c.x.get(), c.v.get(), c.f.get(), c.ptype.get(), c.xhold.get(),
+                                             c.xhold_in.get(), c.P, mass_tab(c), dtf, dt, c.triggersq, hd * hd,
+                                             outer ? 1 : 0, step, c.d_steps.get(), c.h_steps_dev,
+                                             c.d_check_count.get(), c.check_blocks);
   c.x_dirty = true;
 }
 
@@ -262,11 +286,12 @@ void step_end(Ctx& c, double dt, Gate g) {
 // k_step_begin, in the same order, so the state is bitwise the same.  Threads
 // [0, A) are atoms; thread A + 8e + k is node k of element e, which forms its
 // own nodal force in all three dimensions from the element's Gauss forces.
-__global__ void k_step_end_begin(const double* __restrict__ gf, double* __restrict__ f, double* __restrict__ x,
+__global__ __launch_bounds__(CB) void k_step_end_begin(const double* __restrict__ gf, double* __restrict__ f, double* __restrict__ x,
                                  double* __restrict__ v, const int* __restrict__ ptype, const double* __restrict__ xh,
                                  const double* __restrict__ xhi, int A, int E, int P, NodefTab t, MassTab mt,
                                  double dtf, double dtv, double trig, double trig_in, int outer, int step,
-                                 int* __restrict__ d_steps, int* __restrict__ h_steps) {
+                                 int* __restrict__ d_steps, int* __restrict__ h_steps, unsigned* __restrict__ count,
+                                 unsigned target) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   int hit = 0, hit_in = 0;
   if (p < P) {
@@ -319,24 +344,21 @@ __global__ void k_step_end_begin(const double* __restrict__ gf, double* __restri
     hit_in = (ex * ex + ey * ey + ez * ez > trig_in);
   }
   const bool lead = (threadIdx.x & 63) == 0;
-  if (__ballot(hit) && lead) {
-    d_steps[0] = step;
-    h_steps[0] = step;
-  }
-  if (__ballot(hit_in) && lead) {
-    d_steps[1] = step;
-    h_steps[1] = step;
-  }
+  if (__ballot(hit) && lead) mark_step(d_steps, h_steps, step);
+  if (__ballot(hit_in) && lead) mark_step(d_steps + 1, h_steps + 1, step);
+  publish_check(count, target, h_steps + 2, step);
 }
 
 void step_end_begin(Ctx& c, double dt, int next_step, bool outer) {
   ScopedTimer tm(c, "nve");
   const double dtf = 0.5 * dt * c.ftm2v;
   const double hd = 0.5 * c.inner_skin;
-  k_step_end_begin<<<grid_for(c.P, 256), 256, 0, c.stream>>>(
+  const int blocks = grid_for(c.P, CB);
+  c.check_blocks += (unsigned)blocks;
+  k_step_end_begin<<<blocks, CB, 0, c.stream>>>(
       c.gf.get(), c.f.get(), c.x.get(), c.v.get(), c.ptype.get(), c.xhold.get(), c.xhold_in.get(), c.A, c.E, c.P,
       nodef_tab(c), mass_tab(c), dtf, dt, c.triggersq, hd * hd, outer ? 1 : 0, next_step, c.d_steps.get(),
-      c.h_steps_dev);
+      c.h_steps_dev, c.d_check_count.get(), c.check_blocks);
   c.x_dirty = true;
   c.begun_step = next_step;
 }

@@ -462,7 +462,7 @@ void serve(cac_hip_server* s, int me) {
       Ctx& c = s->slots[a.slot].h->c;
       const int ev = a.done + 1 == a.req->r.nsteps ? 1 : 0;
       try {
-        CAC_HIP(hipEventSynchronize(c.ev_check));
+        wait_check(c);
         if (c.h_steps[0] == (int)c.ntimestep) {  // an outer rebuild: the rebuilder takes the step
           a.away = true;
           continue;
