@@ -173,9 +173,12 @@ __device__ __forceinline__ void mark_step(int* d_word, int* h_word, int step) {
   __threadfence_system();
 }
 
-// workgroup size of the fused kick / check kernels: few workgroups, so few
-// counter updates per step
-constexpr int CB = 1024;
+// workgroup size of the fused kick / check kernels: the counter update per
+// workgroup is a device-scope atomic on one address, so fewer, larger
+// workgroups cost less, down to where a launch no longer spreads evenly over
+// the CUs (cac_eam_x8, k_step_end_begin: 27.0 us at 256 threads, 18.9 at 512,
+// 20.4 at 1024; 16.6 at 256 without the count)
+constexpr int CB = 512;
 
 __global__ __launch_bounds__(CB) void k_step_begin(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ f,
                              const int* __restrict__ ptype, const double* __restrict__ xh,
