@@ -113,7 +113,6 @@ struct cac_hip_server {
   std::condition_variable cv_rebuild;
   bool rebuild_stop = false;
   std::vector<std::thread> rebuilders;
-  long rebuilds_away = 0;
   Clock::time_point t0;
 
   // statistics, written by the service thread under mu
@@ -318,7 +317,6 @@ void rebuild_loop(cac_hip_server* s) {
     {
       std::lock_guard<std::mutex> lk(s->mu);
       s->interval_ms.push_back((float)(1000.0 * (t - a.t_last)));
-      s->rebuilds_away++;
     }
     a.t_last = t;
     a.done++;
