@@ -157,6 +157,25 @@ void step_resolve(Ctx& c, double dt, int ev, bool fuse_next) {
   step_close(c, dt, ev, fuse_next);
 }
 
+// the unit constants of `units` style lj or metal (update.cpp:112-184) and
+// its default neighbor skin
+void set_units(Ctx& c, int units) {
+  c.units = units;
+  if (units == CAC_HIP_UNITS_LJ) {
+    c.boltz = 1.0;
+    c.mvv2e = 1.0;
+    c.ftm2v = 1.0;
+    c.nktv2p = 1.0;
+    c.skin = 0.3;
+  } else {
+    c.boltz = 8.617343e-5;
+    c.mvv2e = 1.0364269e-4;
+    c.ftm2v = 1.0 / 1.0364269e-4;
+    c.nktv2p = 1.6021765e6;
+    c.skin = 2.0;
+  }
+}
+
 // A context about to take a new model: box, types, element type, staged
 // arrays, run counters and status words go back to their initial state; the
 // pair style, the neighbor settings and every device buffer are kept, so a
@@ -365,19 +384,7 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     if (const char* e = std::getenv("CAC_HIP_PAIR_SHAPE")) c.pair_shape = std::atoi(e);
     if (const char* e = std::getenv("CAC_HIP_DENS_NT")) c.dens_nt = std::atoi(e) == 1024 ? 1024 : 512;
     CAC_HIP(hipDeviceGetAttribute(&c.ncu, hipDeviceAttributeMultiprocessorCount, device));
-    if (units == CAC_HIP_UNITS_LJ) {
-      c.boltz = 1.0;
-      c.mvv2e = 1.0;
-      c.ftm2v = 1.0;
-      c.nktv2p = 1.0;
-      c.skin = 0.3;
-    } else {
-      c.boltz = 8.617343e-5;
-      c.mvv2e = 1.0364269e-4;
-      c.ftm2v = 1.0 / 1.0364269e-4;
-      c.nktv2p = 1.6021765e6;
-      c.skin = 2.0;
-    }
+    set_units(c, units);
     for (int a = 0; a < 8; a++)
       for (int b = 0; b < 8; b++) {
         int same = 0;

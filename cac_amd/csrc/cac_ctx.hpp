@@ -326,6 +326,7 @@ void step_prepare(Ctx& c);
 void wait_check(Ctx& c);  // until the current step's displacement check has published its verdict
 void step_close(Ctx& c, double dt, int ev, bool fuse_next);
 void reset_model(Ctx& c);  // before loading a new model into a used context
+void set_units(Ctx& c, int units);  // CAC_HIP_UNITS_LJ or CAC_HIP_UNITS_METAL constants
 void compact_live(Ctx& c);  // dense live-site records from the full ones (after a prune)
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
                   std::vector<int>& ucell, std::vector<int>& img);

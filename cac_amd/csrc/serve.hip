@@ -153,10 +153,11 @@ int load(Slot& sl, const Request& q, std::string& msg) {
       msg = cac_hip_last_error(h);
     }
   };
-  if (r.units != c.units) {
-    msg = "a replica's units differ from the server's";
+  if (r.units != CAC_HIP_UNITS_LJ && r.units != CAC_HIP_UNITS_METAL) {
+    msg = "units must be CAC_HIP_UNITS_LJ or CAC_HIP_UNITS_METAL";
     return CAC_HIP_EARG;
   }
+  set_units(c, r.units);  // a slot takes each replica's units
   step(cac_hip_set_box(h, r.lo, r.hi));
   step(cac_hip_set_masses(h, r.ntypes, r.mass));
   if (r.nelements > 0) step(cac_hip_add_etype(h, 1, r.ncell[0], r.ncell[1], r.ncell[2], 2, 2, 2));

@@ -162,7 +162,8 @@ int cac_hip_set_inner_skin(cac_hip_ctx *ctx, double delta);
  * every active replica, then resolves each one's check and queues the rest,
  * so kernels of different replicas overlap on the device.  A slot keeps its
  * buffers when its replica finishes (the next replica reuses them, no device
- * frees while serving).  All replicas share one pair style, set once. */
+ * frees while serving).  All replicas share one pair style, set once; each
+ * replica brings its own units (a slot takes them when it loads it). */
 typedef struct cac_hip_server cac_hip_server;
 
 typedef struct {

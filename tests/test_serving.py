@@ -111,3 +111,32 @@ def test_server_rejects_bad_requests_and_reports_failures():
     with pytest.raises(cac_amd.CacHipError):
         srv.state(rid, d)  # not kept
     srv.close()
+
+
+@pytest.mark.gpu
+def test_lj_replicas_in_lj_units_same_alone_and_in_the_mix():
+    """A server with the LJ pair style takes replicas in LJ units (its slots
+    take each replica's units): four FCC blocks of different size and
+    temperature, 100 mixed steps each, alone and then together on two slots:
+    final positions bitwise equal, final energies equal."""
+    import cac_amd
+    dks = [decks.lj_block(n=n, jitter=0.02, T=T, seed=s) for n, T, s in ((5, 0.1, 1), (6, 0.3, 2), (5, 0.5, 3),
+                                                                           (7, 0.2, 4))]
+    alone = []
+    for d in dks:
+        m = cac_amd.Model(d, "lj", skin=0.3, precision=cac_amd.MIXED)
+        m.setup()
+        th = m.run(100, 0.005)
+        alone.append((m.atoms()[0], th[-1]))
+        m.close()
+    srv = cac_amd.Server(slots=2, pair="lj")
+    ids = [srv.submit(d, 100, 0.005, skin=0.3, keep_state=True) for d in dks]
+    srv.drain()
+    assert srv.stats()["steps"] == 4 * 100
+    for rid, d, (x0, th0) in zip(ids, dks, alone):
+        r = srv.result(rid)
+        assert r["status"] == 0 and r["nsteps"] == 100
+        x, _ = srv.state(rid, d)
+        assert np.array_equal(x, x0)
+        assert r["pe"] == th0[1] and r["ke"] == th0[2], (r, th0)
+    srv.close()
