@@ -373,8 +373,8 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     hs[0] = hs[1] = -1;
     c.h_steps = hs;
     CAC_HIP(hipHostGetDevicePointer((void**)&c.h_steps_dev, hs, 0));
-    c.d_check_count.reserve(1);
-    CAC_HIP(hipMemsetAsync(c.d_check_count.get(), 0, sizeof(unsigned), c.stream));
+    c.d_check_count.reserve(9 * 64);
+    CAC_HIP(hipMemsetAsync(c.d_check_count.get(), 0, sizeof(unsigned) * 9 * 64, c.stream));
     hs[2] = -1;
     CAC_HIP(hipEventCreateWithFlags(&c.ev_seg, hipEventDisableTiming));
     c.units = units;

@@ -282,10 +282,12 @@ struct Ctx {
   DBuf<float4> site32f;           // mixed path: FP32 cell-relative records of every site (for the prune)
   volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
   int* h_steps_dev = nullptr;       // device address of h_steps
-  // the check's completion: its kernels count their workgroups on this device
-  // counter, and the last one writes the step number into h_steps[2]
-  DBuf<unsigned> d_check_count;
-  unsigned check_blocks = 0;        // workgroups counted so far (the running target)
+  // the check's completion: its kernels count their workgroups on per-XCD
+  // device counters and a global one, and the last writes the step number
+  // into h_steps[2]
+  DBuf<unsigned> d_check_count;     // 9 counters, 64 words apart
+  unsigned check_xcd[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // running targets of the XCD counters
+  unsigned check_all = 0;                            // and of the global one
   hipEvent_t ev_seg = nullptr;      // recorded when the model is ready for segmented pair passes
   double* h_stage = nullptr;        // pinned staging of the per-step host handover
   size_t h_stage_n = 0;

@@ -158,12 +158,37 @@ void nodal_assembly(Ctx& c) {
 // expects) fences and writes the step number into the host-mapped `done`
 // word, which the host polls instead of waiting on an event recorded in the
 // stream.
-__device__ __forceinline__ void publish_check(unsigned* count, unsigned target, int* done, int step) {
+// The count has two levels so that few workgroups update any one address:
+// workgroup b counts itself on the counter of XCD b % 8, and the workgroup
+// that completes an XCD's count then counts that XCD on a global counter.
+// Counters sit 256 bytes apart; the host passes the running targets.
+struct CheckTargets {
+  unsigned xcd[8];  // workgroups counted on each XCD counter so far
+  unsigned all;     // XCD counts completed so far
+};
+constexpr int CHECK_STRIDE = 64;  // unsigned words between counters
+
+__device__ __forceinline__ void publish_check(unsigned* count, const CheckTargets& t, int* done, int step) {
   __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(count, 1u) == target - 1u) {
-    __threadfence_system();
-    *reinterpret_cast<volatile int*>(done) = step;
+  if (threadIdx.x != 0) return;
+  const int x = blockIdx.x % 8;
+  if (atomicAdd(count + CHECK_STRIDE * x, 1u) != t.xcd[x] - 1u) return;
+  if (atomicAdd(count + CHECK_STRIDE * 8, 1u) != t.all - 1u) return;
+  __threadfence_system();
+  *reinterpret_cast<volatile int*>(done) = step;
+}
+
+// the running targets after one more launch of `blocks` workgroups
+static CheckTargets next_targets(Ctx& c, int blocks) {
+  for (int x = 0; x < 8; x++) {
+    const unsigned nx = (unsigned)((blocks - x + 7) / 8);  // workgroups b < blocks with b % 8 == x
+    c.check_xcd[x] += nx;
+    if (nx) c.check_all++;
   }
+  CheckTargets t;
+  for (int x = 0; x < 8; x++) t.xcd[x] = c.check_xcd[x];
+  t.all = c.check_all;
+  return t;
 }
 
 // the step words of a wave whose points hit a trigger
@@ -173,18 +198,17 @@ __device__ __forceinline__ void mark_step(int* d_word, int* h_word, int step) {
   __threadfence_system();
 }
 
-// workgroup size of the fused kick / check kernels: the counter update per
-// workgroup is a device-scope atomic on one address, so fewer, larger
-// workgroups cost less, down to where a launch no longer spreads evenly over
-// the CUs (cac_eam_x8, k_step_end_begin: 27.0 us at 256 threads, 18.9 at 512,
-// 20.4 at 1024; 16.6 at 256 without the count)
+// workgroup size of the fused kick / check kernels (cac_eam_x8,
+// k_step_end_begin: 16.4 us with the two-level count; with a single counter
+// 27.0 us at 256 threads, 18.9 at 512 and 20.4 at 1024; 16.6 at 256 without
+// any count)
 constexpr int CB = 512;
 
 __global__ __launch_bounds__(CB) void k_step_begin(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ f,
                              const int* __restrict__ ptype, const double* __restrict__ xh,
                              const double* __restrict__ xhi, int P, MassTab mt, double dtf, double dtv, double trig,
                              double trig_in, int outer, int step, int* __restrict__ d_steps,
-                             int* __restrict__ h_steps, unsigned* __restrict__ count, unsigned target) {
+                             int* __restrict__ h_steps, unsigned* __restrict__ count, CheckTargets target) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   int hit = 0, hit_in = 0;
   if (p < P) {
@@ -215,11 +239,11 @@ void step_begin(Ctx& c, double dt, int step, bool outer) {
   const double dtf = 0.5 * dt * c.ftm2v;
   const double hd = 0.5 * c.inner_skin;
   const int blocks = grid_for(c.P, CB);
-  c.check_blocks += (unsigned)blocks;
+  const CheckTargets t = next_targets(c, blocks);
   k_step_begin<<<blocks, CB, 0, c.stream>>>(c.x.get(), c.v.get(), c.f.get(), c.ptype.get(), c.xhold.get(),
                                              c.xhold_in.get(), c.P, mass_tab(c), dtf, dt, c.triggersq, hd * hd,
                                              outer ? 1 : 0, step, c.d_steps.get(), c.h_steps_dev,
-                                             c.d_check_count.get(), c.check_blocks);
+                                             c.d_check_count.get(), t);
   c.x_dirty = true;
 }
 
@@ -294,7 +318,7 @@ __global__ __launch_bounds__(CB) void k_step_end_begin(const double* __restrict_
                                  const double* __restrict__ xhi, int A, int E, int P, NodefTab t, MassTab mt,
                                  double dtf, double dtv, double trig, double trig_in, int outer, int step,
                                  int* __restrict__ d_steps, int* __restrict__ h_steps, unsigned* __restrict__ count,
-                                 unsigned target) {
+                                 CheckTargets target) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   int hit = 0, hit_in = 0;
   if (p < P) {
@@ -357,11 +381,11 @@ void step_end_begin(Ctx& c, double dt, int next_step, bool outer) {
   const double dtf = 0.5 * dt * c.ftm2v;
   const double hd = 0.5 * c.inner_skin;
   const int blocks = grid_for(c.P, CB);
-  c.check_blocks += (unsigned)blocks;
+  const CheckTargets t = next_targets(c, blocks);
   k_step_end_begin<<<blocks, CB, 0, c.stream>>>(
       c.gf.get(), c.f.get(), c.x.get(), c.v.get(), c.ptype.get(), c.xhold.get(), c.xhold_in.get(), c.A, c.E, c.P,
       nodef_tab(c), mass_tab(c), dtf, dt, c.triggersq, hd * hd, outer ? 1 : 0, next_step, c.d_steps.get(),
-      c.h_steps_dev, c.d_check_count.get(), c.check_blocks);
+      c.h_steps_dev, c.d_check_count.get(), t);
   c.x_dirty = true;
   c.begun_step = next_step;
 }
