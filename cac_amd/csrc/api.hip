@@ -23,6 +23,7 @@ Ctx::~Ctx() {
   if (h_red) (void)hipHostFree(h_red);
   if (h_flags) (void)hipHostFree(h_flags);
   if (h_steps) (void)hipHostFree(const_cast<int*>(h_steps));
+  if (h_pub) (void)hipHostFree(const_cast<int*>(h_pub));
   if (h_stage) (void)hipHostFree(h_stage);
   if (ev_seg) (void)hipEventDestroy(ev_seg);
   for (auto& e : ev_pool)
@@ -104,27 +105,56 @@ bool step_issue(Ctx& c, double dt, int ev) {
   return true;
 }
 
-// Poll the host-mapped word the check's last workgroup writes.  After 1 ms
-// the stream is also queried (then every 0.1 ms), so a failed launch is an
-// error, not an endless wait; before that the poll touches no runtime call,
-// which would take the runtime's locks.
-void wait_check(Ctx& c) {
+// Poll a host-mapped word the device writes until it holds `want`.  After
+// 1 ms the stream is also queried (then every 0.1 ms), so a failed launch is
+// an error, not an endless wait; before that the poll touches no runtime
+// call, which would take the runtime's locks.
+static void poll_word(Ctx& c, volatile int* w, int want, const char* what) {
   using clk = std::chrono::steady_clock;
-  const int N = (int)c.ntimestep;
-  volatile int* done = c.h_steps + 2;
-  if (*done == N) return;
-  const auto t0 = clk::now();
-  auto next_query = t0 + std::chrono::milliseconds(1);
-  for (unsigned spin = 1; *done != N; spin++) {
+  if (*w == want) return;
+  auto next_query = clk::now() + std::chrono::milliseconds(1);
+  for (unsigned spin = 1; *w != want; spin++) {
     if ((spin & 1023u) != 0) continue;
     const auto t = clk::now();
     if (t < next_query) continue;
     next_query = t + std::chrono::microseconds(100);
     const hipError_t q = hipStreamQuery(c.stream);
-    if (q != hipSuccess && q != hipErrorNotReady) throw HipError{q, "displacement check"};
-    if (q == hipSuccess && *done != N)
-      throw HipError{hipErrorUnknown, "displacement check ended without publishing its verdict"};
+    if (q != hipSuccess && q != hipErrorNotReady) throw HipError{q, what};
+    if (q == hipSuccess && *w != want) throw HipError{hipErrorUnknown, std::string(what) + ": the device never wrote"};
   }
+}
+
+// the host-mapped word the check's last workgroup writes
+void wait_check(Ctx& c) { poll_word(c, c.h_steps + 2, (int)c.ntimestep, "displacement check"); }
+
+// One thread copies the values, fences at system scope, then writes the
+// sequence number the host polls for.
+__global__ void k_publish_ints(const int* __restrict__ src, int n, const int* __restrict__ a,
+                               const int* __restrict__ b, int* pub, int seq) {
+  if (threadIdx.x != 0) return;
+  if (a)
+    pub[1] = *a + *b;
+  else
+    for (int i = 0; i < n; i++) pub[1 + i] = src[i];
+  __threadfence_system();
+  *reinterpret_cast<volatile int*>(pub) = seq;
+}
+
+void read_device_ints(Ctx& c, const int* src, int n, int* dst) {
+  if (n < 1 || n > 8) throw HipError{hipErrorInvalidValue, "read_device_ints: 1 to 8 values"};
+  const int seq = ++c.pub_seq;
+  k_publish_ints<<<1, 64, 0, c.stream>>>(src, n, nullptr, nullptr, c.h_pub_dev, seq);
+  CAC_HIP(hipGetLastError());
+  poll_word(c, c.h_pub, seq, "device values");
+  for (int i = 0; i < n; i++) dst[i] = c.h_pub[1 + i];
+}
+
+int read_device_sum(Ctx& c, const int* a, const int* b) {
+  const int seq = ++c.pub_seq;
+  k_publish_ints<<<1, 64, 0, c.stream>>>(nullptr, 0, a, b, c.h_pub_dev, seq);
+  CAC_HIP(hipGetLastError());
+  poll_word(c, c.h_pub, seq, "device count");
+  return c.h_pub[1];
 }
 
 // step_resolve in three parts, so that a caller can run the pair passes of
@@ -373,6 +403,11 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     hs[0] = hs[1] = -1;
     c.h_steps = hs;
     CAC_HIP(hipHostGetDevicePointer((void**)&c.h_steps_dev, hs, 0));
+    int* hp = nullptr;
+    CAC_HIP(hipHostMalloc((void**)&hp, sizeof(int) * 16, hipHostMallocMapped | hipHostMallocCoherent));
+    for (int i = 0; i < 16; i++) hp[i] = 0;
+    c.h_pub = hp;
+    CAC_HIP(hipHostGetDevicePointer((void**)&c.h_pub_dev, hp, 0));
     c.d_check_count.reserve(9 * 64);
     CAC_HIP(hipMemsetAsync(c.d_check_count.get(), 0, sizeof(unsigned) * 9 * 64, c.stream));
     hs[2] = -1;

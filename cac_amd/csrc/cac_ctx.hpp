@@ -286,6 +286,11 @@ struct Ctx {
   // device counters and a global one, and the last writes the step number
   // into h_steps[2]
   DBuf<unsigned> d_check_count;     // 9 counters, 64 words apart
+  // host-mapped words through which a one-workgroup kernel hands small
+  // device values (counts, status words) to the host: [0] sequence, [1..] values
+  volatile int* h_pub = nullptr;
+  int* h_pub_dev = nullptr;
+  int pub_seq = 0;
   unsigned check_xcd[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // running targets of the XCD counters
   unsigned check_all = 0;                            // and of the global one
   hipEvent_t ev_seg = nullptr;      // recorded when the model is ready for segmented pair passes
@@ -326,6 +331,11 @@ void step_resolve(Ctx& c, double dt, int ev, bool fuse_next = false);
 // pair passes, step_close (nodal forces and the closing kick)
 void step_prepare(Ctx& c);
 void wait_check(Ctx& c);  // until the current step's displacement check has published its verdict
+// small device values read back through the host-mapped words, in stream
+// order (no copy engine, no stream synchronization): n <= 8 ints, or the sum
+// of two ints
+void read_device_ints(Ctx& c, const int* src, int n, int* dst);
+int read_device_sum(Ctx& c, const int* a, const int* b);
 void step_close(Ctx& c, double dt, int ev, bool fuse_next);
 void reset_model(Ctx& c);  // before loading a new model into a used context
 void set_units(Ctx& c, int units);  // CAC_HIP_UNITS_LJ or CAC_HIP_UNITS_METAL constants

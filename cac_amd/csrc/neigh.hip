@@ -437,8 +437,8 @@ __global__ __launch_bounds__(256) void k_remap_rows(int* __restrict__ row_site, 
 }
 
 static void read_flags(Ctx& c) {
-  CAC_HIP(hipMemcpyAsync(c.h_flags, c.d_flags.get(), sizeof(ErrorFlags), hipMemcpyDeviceToHost, c.stream));
-  CAC_HIP(hipStreamSynchronize(c.stream));
+  static_assert(sizeof(ErrorFlags) == 8 * sizeof(int), "ErrorFlags is eight ints");
+  read_device_ints(c, reinterpret_cast<const int*>(c.d_flags.get()), 8, reinterpret_cast<int*>(c.h_flags));
 }
 
 void build_neighbor(Ctx& c) {

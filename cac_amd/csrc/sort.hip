@@ -27,13 +27,7 @@ void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total) {
   CAC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, c.stream));
   c.scan_tmp.reserve(tmp / sizeof(int) + 16);
   CAC_HIP(hipcub::DeviceScan::ExclusiveSum(c.scan_tmp.get(), tmp, in, out, n, c.stream));
-  if (total) {
-    int last_in = 0, last_out = 0;
-    CAC_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CAC_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CAC_HIP(hipStreamSynchronize(c.stream));
-    *total = last_in + last_out;
-  }
+  if (total) *total = read_device_sum(c, in + n - 1, out + n - 1);
 }
 
 // slot i opens the cells after the previous slot's cell up to its own
