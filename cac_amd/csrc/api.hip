@@ -261,6 +261,7 @@ int guarded(cac_hip_ctx* h, F&& fn) {
       if (e.where.find("overflow") != std::string::npos) return CAC_HIP_EOVERFLOW;
       if (e.where.find("Non-numeric") != std::string::npos) return CAC_HIP_ENAN;
       if (e.where.find("deformed") != std::string::npos) return CAC_HIP_EDEFORM;
+      if (e.where.find("larger than the periodic box") != std::string::npos) return CAC_HIP_EDEFORM;
       return CAC_HIP_EARG;
     }
     if (e.code == hipErrorOutOfMemory) {

@@ -175,6 +175,7 @@ int code_of(const HipError& e) {
     if (e.where.find("overflow") != std::string::npos) return CAC_HIP_EOVERFLOW;
     if (e.where.find("Non-numeric") != std::string::npos) return CAC_HIP_ENAN;
     if (e.where.find("deformed") != std::string::npos) return CAC_HIP_EDEFORM;
+    if (e.where.find("larger than the periodic box") != std::string::npos) return CAC_HIP_EDEFORM;
     return CAC_HIP_EARG;
   }
   return e.code == hipErrorOutOfMemory ? CAC_HIP_EOOM : CAC_HIP_EHIP;

@@ -25,7 +25,7 @@ enum {
   CAC_HIP_EARG = 1,      /* bad argument (maps to error->all) */
   CAC_HIP_EOVERFLOW = 2, /* a neighbor row exceeds the per-point capacity ("neigh_modify one") */
   CAC_HIP_ENAN = 3,      /* non-numeric coordinates (domain.cpp:541-546, nbin.cpp:130-131) */
-  CAC_HIP_EDEFORM = 4,   /* element deformed more than 2x (element.cpp:1452-1461) */
+  CAC_HIP_EDEFORM = 4,   /* element deformed more than 2x (element.cpp:1452-1461), or grown larger than the box */
   CAC_HIP_EOOM = 5,      /* device allocation failed */
   CAC_HIP_EHIP = 6,      /* HIP runtime error */
   CAC_HIP_ESTATE = 7,    /* call out of order (e.g. compute before a neighbor build) */
