@@ -49,6 +49,7 @@ WORKLOADS = {
 SKIN, DT = 2.0, 0.001
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "eam_force_mixed_cac_eam_x8_pmc.md")
+VALU_SUMMARY = os.path.join(ROOT, "profiles", "pair_passes_mixed_cac_eam_x8_valu.md")
 
 
 def force_vector(m):
@@ -151,10 +152,11 @@ def force_bytes(P, inner_mean, live_sites):
     return P * (36.0 + 4.0 * inner_mean) + 16.0 * live_sites
 
 
-def traffic_from_summary():
-    """HBM bytes per force launch measured with FETCH_SIZE / WRITE_SIZE (committed summary), or None."""
+def traffic_from_summary(path=TRAFFIC_SUMMARY):
+    """The JSON block of a committed counter summary (HBM bytes per force launch measured with FETCH_SIZE /
+    WRITE_SIZE, or VALU instructions per force launch), or None."""
     try:
-        text = open(TRAFFIC_SUMMARY).read()
+        text = open(path).read()
     except OSError:
         return None
     m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
@@ -289,6 +291,17 @@ def main():
     bytes_force = force_bytes(P, nbar_in, live)
     achieved = bytes_force / (force_us * 1e-6) / 1e9
     tr = traffic_from_summary() if args.workload == "cac_eam_x8" else None
+    vs = traffic_from_summary(VALU_SUMMARY) if args.workload == "cac_eam_x8" else None
+    valu = None
+    if vs:
+        # VALU instructions per inner-list entry are a property of the code
+        # (counted once under rocprofv3); scaled to this run's lists and
+        # divided by the live kernel time
+        insts = vs["valu_insts_per_launch"] / vs["entries_per_launch"] * P * nbar_in
+        rate = insts / (force_us * 1e-6)
+        valu = {"insts_per_launch": insts, "achieved_insts_per_s": rate,
+                "peak_insts_per_s": vs["peak_valu_insts_per_s"], "frac": rate / vs["peak_valu_insts_per_s"],
+                "source": os.path.relpath(VALU_SUMMARY, ROOT)}
 
     drift = None
     if args.drift_steps > 0:
@@ -340,7 +353,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tr["bytes_per_launch"] if tr else None,
                      "algorithmic_bytes": bytes_force,
-                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if tr else None},
+                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if tr else None,
+                     "valu": valu},
         "nve_drift": drift,
         "serving": serving,
         "cpu_baseline": cpu,
