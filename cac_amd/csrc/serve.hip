@@ -96,7 +96,7 @@ struct cac_hip_server {
   std::vector<Slot> slots;
   bool pair_set = false, submitted = false;
 
-  std::mutex mu;
+  mutable std::mutex mu;  // mutable: the const accessors lock it too
   std::condition_variable cv_work, cv_ready, cv_done;
   std::deque<std::unique_ptr<Request>> queue;
   std::vector<std::vector<Active>> ready;  // per stepping thread: loaded, not yet stepped
@@ -575,7 +575,16 @@ int cac_hip_server_destroy(cac_hip_server* s) {
   return CAC_HIP_OK;
 }
 
-const char* cac_hip_server_last_error(const cac_hip_server* s) { return s ? s->err.c_str() : "null server"; }
+// The service threads may replace the message at any time, so the caller gets
+// a copy taken under the lock, held per calling thread: the pointer stays
+// valid until that thread calls again.
+const char* cac_hip_server_last_error(const cac_hip_server* s) {
+  if (!s) return "null server";
+  thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(s->mu);
+  copy = s->err;
+  return copy.c_str();
+}
 
 int cac_hip_server_pair_eam(cac_hip_server* s, int nrho, double drho, int nr, double dr, double cut,
                             const double* frho, const double* rhor, const double* z2r) {

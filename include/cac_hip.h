@@ -198,7 +198,8 @@ typedef struct {
 /* slots: concurrent replicas (1..256); hbm_budget_bytes <= 0: 3/4 of free HBM */
 int cac_hip_server_create(cac_hip_server **srv, int device, int slots, double hbm_budget_bytes);
 int cac_hip_server_destroy(cac_hip_server *srv);
-/* the last error: of a server call, or the message of the last replica that failed */
+/* the last error: of a server call, or the message of the last replica that failed;
+   the string is the calling thread's copy, valid until that thread calls again */
 const char *cac_hip_server_last_error(const cac_hip_server *srv);
 /* the pair style of every replica (as cac_hip_pair_eam / cac_hip_pair_lj) */
 int cac_hip_server_pair_eam(cac_hip_server *srv, int nrho, double drho, int nr, double dr, double cut,
