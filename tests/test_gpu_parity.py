@@ -306,6 +306,42 @@ def test_mixed_nve_drift_within_twice_fp64():
     assert d32 <= 2 * d64 + floor, res
 
 
+def test_nve_drift_over_1e4_steps_matches_oracle():
+    """NVE on a 500-atom EAM block at 300 K, 10,000 steps of 1 fs, from the same
+    state in the oracle, the FP64 path and the mixed path.  The trajectories
+    part after a few hundred steps (the summation orders differ, and MD
+    amplifies differences), so what must match is energy conservation: E(0)
+    equal, and each run's largest excursion |E(t) - E(0)| and drift (slope of a
+    linear fit of E(t), per atom) within twice the oracle's (the mixed run's
+    also within twice FP64's), with a floor well below a faulty integrator's."""
+    import cac_amd
+    deck = decks.eam_block(n=5, seed=31)
+    o = oracle_model(deck, "eam", 2.0)
+    o.setup()
+    runs = {"oracle": o.run(10000, 0.001, thermo_every=100)}
+    o.close()
+    for name, prec in (("fp64", cac_amd.FP64), ("mixed", cac_amd.MIXED)):
+        g = gpu_model(deck, "eam", 2.0, precision=prec)
+        g.setup()
+        runs[name] = g.run(10000, 0.001, thermo_every=100)
+        g.close()
+    stats = {}
+    for name, th in runs.items():
+        t, e = th[:, 0] * 0.001, th[:, 3]
+        assert len(e) == 101 and np.all(np.isfinite(e)), name
+        stats[name] = (float(np.max(np.abs(e - e[0])) / abs(e[0])), abs(float(np.polyfit(t, e, 1)[0])) / deck.natoms)
+    e0 = runs["oracle"][0, 3]
+    assert abs(runs["fp64"][0, 3] - e0) <= 1e-12 * abs(e0)
+    assert abs(runs["mixed"][0, 3] - e0) <= 1e-6 * abs(e0)
+    xo, do = stats["oracle"]
+    x_floor, d_floor = 1e-7, 2e-7  # relative excursion; eV per atom per ps
+    for name in ("fp64", "mixed"):
+        x, d = stats[name]
+        assert x <= 2 * xo + x_floor and d <= 2 * do + d_floor, stats
+    assert stats["mixed"][0] <= 2 * stats["fp64"][0] + x_floor, stats
+    assert stats["mixed"][1] <= 2 * stats["fp64"][1] + d_floor, stats
+
+
 # ------------------------------------------------------------------ host handover, closed forms, launch shapes
 
 def test_host_handover_matches_oracle():
