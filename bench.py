@@ -22,7 +22,8 @@ often as it does, so the per-step cost grows with --steps; the default
 Prints one JSON line.  `--profile-run` instead runs only the mixed model for
 --steps steps and then a 1 GiB device-to-device copy, the calibration kernel
 of the HBM traffic counters; it is the program the committed rocprofv3
-summaries were taken on.
+summaries were taken on.  `--handover` instead times the per-call drop-in,
+with host positions in and forces out every step (PCIe inside the step).
 """
 
 import argparse
@@ -97,6 +98,50 @@ def nve_drift(cac_amd, deck, tab, precision, steps, every=100):
     slope = float(np.polyfit(t, e, 1)[0]) / deck.atom_equivalents
     return {"drift_eV_per_atom_ps": slope, "max_rel_excursion": float(np.max(np.abs(e - e[0])) / abs(e[0])),
             "e0": float(e[0]), "e_end": float(e[-1])}
+
+
+def handover_run(cac_amd, deck, tab, steps=50, warmup=5):
+    """The per-call drop-in (a caller that keeps its integrator on the host):
+    every step host positions go in (sync_positions), the rebuild decision is
+    made (decide), forces are computed and come back out (download_forces), so
+    PCIe is inside the step.  Positions stay fixed, so the host integrator's own
+    cost is left out.  Returns the rate and the host time of each call; compute
+    only queues the passes, whose device time shows up in the force download,
+    which waits for them."""
+    import numpy as np
+    P = deck.atom_equivalents
+    x = np.array(deck.x, dtype=np.float64)
+    nx = np.array(deck.nodex, dtype=np.float64).reshape(-1, 8, 3)
+    out = {}
+    for name, prec in (("mixed", cac_amd.MIXED), ("fp64", cac_amd.FP64)):
+        m = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=prec)
+        m.setup()
+        for _ in range(warmup):
+            m.sync_positions(x, nx)
+            m.decide()
+            m.compute(0, 0)
+            m.forces()
+        t = [0.0] * 4
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a = time.perf_counter()
+            m.sync_positions(x, nx)
+            b = time.perf_counter()
+            m.decide()
+            c = time.perf_counter()
+            m.compute(0, 0)
+            d = time.perf_counter()
+            m.forces()
+            e = time.perf_counter()
+            for i, dt in enumerate((b - a, c - b, d - c, e - d)):
+                t[i] += dt
+        wall = time.perf_counter() - t0
+        m.close()
+        out[name] = {"value": P * steps / wall, "ms_per_step": 1000.0 * wall / steps,
+                     "ms_sync_positions": 1000.0 * t[0] / steps, "ms_decide": 1000.0 * t[1] / steps,
+                     "ms_compute_queue": 1000.0 * t[2] / steps, "ms_download_forces": 1000.0 * t[3] / steps}
+    return {"unit": "atom-eq steps/s", "steps": steps, "host_to_device_bytes": 24 * P,
+            "device_to_host_bytes": 24 * P + 24 * 8 * deck.nelements, **out}
 
 
 def serve_run(cac_amd, decks, tab, n, steps, rates, slots, seed=77):
@@ -204,6 +249,8 @@ def main():
                          "and a saturating one for capacity")
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
+    ap.add_argument("--handover", action="store_true",
+                    help="run only the per-call drop-in (host positions in, forces out every step) on the workload")
     ap.add_argument("--profile-run", action="store_true",
                     help="mixed model only, --steps steps, then a 1 GiB copy (program of the rocprofv3 summaries)")
     args = ap.parse_args()
@@ -226,6 +273,9 @@ def main():
         return
     deck = build_deck(decks)
     P = deck.atom_equivalents
+    if args.handover:
+        print(json.dumps({"handover": handover_run(cac_amd, deck, tab), "workload": args.workload}))
+        return
 
     if args.profile_run:
         m = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.MIXED)
