@@ -375,7 +375,8 @@ def main():
                          f"after its setup; setup (neighbor build + first force, {t_setup:.1f} s) excluded"}
 
     line = {
-        "metric": "atom-steps/s (mixed vs FP64 @ <1e-5 rel force err)",
+        # the served p50/p99 step times are the "serving" object of the same line
+        "metric": "atom-steps/s (mixed vs FP64 @ <1e-5 rel force err); served p50/p99 step ms",
         "value": value,
         "unit": "atom-eq steps/s",
         "n_gpus": 1,
