@@ -88,6 +88,7 @@ def lib():
             "cac_hip_neigh_export": [vp, C.POINTER(C.c_long), vp, vp, vp, vp, vp],
             "cac_hip_compute": [vp, i, i],
             "cac_hip_get_energy": [vp, C.POINTER(d), _dp],
+            "cac_hip_get_peratom": [vp, vp, vp, vp, vp],
             "cac_hip_nve_initial": [vp, d],
             "cac_hip_nve_final": [vp, d],
             "cac_hip_setup": [vp, i, i],
@@ -236,6 +237,15 @@ class Model:
         vir = np.zeros(6)
         self._chk(self.L.cac_hip_get_energy(self.h, C.byref(e), vir))
         return e.value, vir
+
+    def peratom(self, energy=True, virial=True):
+        """Per-point tallies of the last compute: eatom [A], enode [E,8],
+        vatom [A,6], vnode [E,8,6] (None where not asked for)."""
+        A, E = self.deck.natoms, self.deck.nelements
+        ea, en = (np.zeros(A), np.zeros((E, 8))) if energy else (None, None)
+        va, vn = (np.zeros((A, 6)), np.zeros((E, 8, 6))) if virial else (None, None)
+        self._chk(self.L.cac_hip_get_peratom(self.h, _p(ea), _p(en), _p(va), _p(vn)))
+        return dict(eatom=ea, enode=en, vatom=va, vnode=vn)
 
     def thermo(self):
         v = [C.c_double() for _ in range(4)]

@@ -233,6 +233,7 @@ void reset_model(Ctx& c) {
   c.entries = -1;
   c.eng_vdwl = 0.0;
   for (double& v : c.virial) v = 0.0;
+  c.peratom_flags = 0;
   c.timers.clear();
   CAC_HIP(hipMemsetAsync(c.d_flags.get(), 0, sizeof(ErrorFlags), c.stream));
   CAC_HIP(hipMemsetAsync(c.d_steps.get(), 0xFF, sizeof(int) * 2, c.stream));
@@ -828,9 +829,39 @@ int cac_hip_neigh_export(cac_hip_ctx* h, long* n, int* row, int* kind, int* owne
 int cac_hip_compute(cac_hip_ctx* h, int eflag, int vflag) {
   return guarded(h, [&](Ctx& c) -> int {
     if (!c.built) return fail(h, CAC_HIP_ESTATE, "neighbor list not built");
+    if (eflag < 0 || eflag > 3 || vflag < 0 || vflag > 15)
+      return fail(h, CAC_HIP_EARG, "eflag must be 0..3 and vflag 0..15 (force.h:11-19)");
     CAC_HIP(hipSetDevice(c.device));
     compute_forces(c, eflag, vflag);
     CAC_HIP(hipStreamSynchronize(c.stream));
+    return CAC_HIP_OK;
+  });
+}
+
+// Pair::eatom / enode / vatom / vnode of the last force computation
+// (pair.h:13-17): per point, ordered as the caller's arrays, from the [7][P]
+// per-point rows the force pass wrote (point A + 8e + k is node k of element e)
+int cac_hip_get_peratom(cac_hip_ctx* h, double* eatom, double* enode, double* vatom, double* vnode) {
+  return guarded(h, [&](Ctx& c) -> int {
+    if ((eatom || enode) && !(c.peratom_flags & 1))
+      return fail(h, CAC_HIP_ESTATE, "per-atom energy was not computed (eflag & ENERGY_ATOM)");
+    if ((vatom || vnode) && !(c.peratom_flags & 2))
+      return fail(h, CAC_HIP_ESTATE, "per-atom virial was not computed (vflag & VIRIAL_ATOM or VIRIAL_CENTROID)");
+    const int A = c.A, P = c.P;
+    const size_t E8 = (size_t)NPE * c.E;
+    const bool want_v = vatom || vnode;
+    std::vector<double> b((size_t)(want_v ? 7 : 1) * P);
+    CAC_HIP(hipStreamSynchronize(c.stream));
+    if (P) CAC_HIP(hipMemcpy(b.data(), c.peratom.get(), sizeof(double) * b.size(), hipMemcpyDeviceToHost));
+    if (eatom) std::memcpy(eatom, b.data(), sizeof(double) * A);
+    if (enode) std::memcpy(enode, b.data() + A, sizeof(double) * E8);
+    for (int k = 0; k < 6 && want_v; k++) {
+      const double* src = b.data() + (size_t)(k + 1) * P;
+      if (vatom)
+        for (int i = 0; i < A; i++) vatom[6 * (size_t)i + k] = src[i];
+      if (vnode)
+        for (size_t q = 0; q < E8; q++) vnode[6 * q + k] = src[A + q];
+    }
     return CAC_HIP_OK;
   });
 }
@@ -862,6 +893,8 @@ int cac_hip_setup(cac_hip_ctx* h, int eflag, int vflag) {
   return guarded(h, [&](Ctx& c) -> int {
     int rc = require_model(c);
     if (rc) return rc;
+    if (eflag < 0 || eflag > 3 || vflag < 0 || vflag > 15)
+      return fail(h, CAC_HIP_EARG, "eflag must be 0..3 and vflag 0..15 (force.h:11-19)");
     CAC_HIP(hipSetDevice(c.device));
     neighbor_init(c);
     build_neighbor(c);

@@ -271,6 +271,8 @@ struct Ctx {
   // energy / virial
   double eng_vdwl = 0.0, virial[6] = {0, 0, 0, 0, 0, 0};
   DBuf<double> erow;      // [7][R]: energy then 6 virial components per row
+  DBuf<double> peratom;   // [7][P]: per-point energy and virial of the last pass that asked for them
+  int peratom_flags = 0;  // 1: per-point energies, 2: per-point virials, of the last pair passes
   DBuf<double> partials;  // block partial sums
   double* h_red = nullptr;  // pinned host scratch for reductions
   ErrorFlags* h_flags = nullptr;  // pinned mirror of the device flags
@@ -349,6 +351,7 @@ void site_positions(Ctx& c);  // FP64 records of every site from its descriptor 
 void order_rows(Ctx& c);      // rows in Morton order of their cells
 
 // pair.hip
+UcellGrid ucell_grid(const Ctx& c);  // unit-cell grid of the element type, for hex_shape_at
 // positions of every live site; full = also the FP64 records (always kept in
 // FP64 mode; the mixed path needs them only to build or prune lists)
 void site_refresh(Ctx& c, bool full, Gate g = Gate());

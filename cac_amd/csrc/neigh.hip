@@ -307,6 +307,98 @@ __global__ void k_rows(const int* __restrict__ sdesc, const int* __restrict__ is
   row_site[r] = k;
 }
 
+// The reference's form of a squared pair distance (npair_half_bin_newton.cpp:
+// 89-141, 194-199, 284-289, 346-364), for a pair whose squared distance lies
+// within a relative 1e-9 of the list cutoff; the fast form of the list scan
+// decides every other pair, where the two cannot disagree.
+//
+// The reference subtracts from a row point's own position its neighbor's
+// ghost coordinate: atom x + T L, or for an interpolated atom N_k (x_k + T L)
+// node by node, T L being the box shift of the ghost copy
+// (atom_vec_atomic.cpp:280-345, element_vec_cac.cpp:596-821).  Gauss points
+// sit on nodes (N is 0 or 1), so a Gauss point's position is its node's
+// coordinate exactly, as a row and as a neighbor.  The site records hold each
+// site wrapped into the box by its own shift W (an interpolated atom as
+// sum_k N_k x_k + W L), so the pair's ghost shift is T = W_site + t_entry -
+// W_row, and the fast form rounds differently.
+//
+// A pair of two points is tested once in the reference, from the row that
+// holds it in the half list: an atom-Gauss pair from the atom's row (Gauss
+// rows hold no atoms); a Gauss-Gauss pair from the lower element index, or
+// the lower Gauss cell within one element, and across the periodic boundary
+// from the element whose neighbor's ghost centre lies above-right of its own
+// (z, then y, then x; npair_half_bin_newton.cpp:346-357).  An atom-atom pair
+// follows the reference's bins, which this build does not form: it is tested
+// from the row at hand (with T = 0 both directions round alike).
+constexpr double NEAR_CUT_REL = 1e-9;
+
+struct RefOrder {
+  const double* ex;  // element centres [3][E], wrapped as Domain::pbc leaves them
+  int E;
+  int n2g[8];        // node -> Gauss cell
+};
+
+// does the reference store the Gauss-Gauss pair (element ei cell gi, element
+// ej cell gj seen through ghost shift T) in ei's row?
+__device__ __forceinline__ bool gauss_pair_in_row(const RefOrder& ro, const double* L, int ei, int gi, int ej, int gj,
+                                                  const int* T) {
+  if (T[0] == 0 && T[1] == 0 && T[2] == 0) return ei < ej || (ei == ej && gi < gj);
+  double ci[3], cj[3];
+  for (int d = 0; d < 3; d++) {
+    ci[d] = ro.ex[(size_t)d * ro.E + ei];
+    cj[d] = ro.ex[(size_t)d * ro.E + ej] + (double)T[d] * L[d];
+  }
+  if (cj[2] < ci[2]) return false;
+  if (cj[2] == ci[2]) {
+    if (cj[1] < ci[1]) return false;
+    if (cj[1] == ci[1] && cj[0] < ci[0]) return false;
+  }
+  return true;
+}
+
+__device__ __noinline__ double reference_rsq(const double* __restrict__ x, int P, int A, const UcellGrid& ug,
+                                             const RefOrder& ro, const double* L, int prow, int wrow, int dsite,
+                                             int wsite, unsigned tcode) {
+#pragma clang fp contract(off)
+  int T[3];
+  T[0] = (wsite % 3 - 1) + ((int)(tcode % 3u) - 1) - (wrow % 3 - 1);
+  T[1] = ((wsite / 3) % 3 - 1) + ((int)((tcode / 3u) % 3u) - 1) - ((wrow / 3) % 3 - 1);
+  T[2] = (wsite / 9 - 1) + ((int)(tcode / 9u) - 1) - (wrow / 9 - 1);
+  double del[3];
+  if (dsite >= 0) {
+    // two points: test from the row the reference keeps the pair in
+    int from = prow, to = dsite, sgn = 1;
+    if (prow >= A && dsite < A) {
+      from = dsite;  // atom-Gauss: the atom's row
+      to = prow;
+      sgn = -1;
+    } else if (prow >= A && dsite >= A) {
+      const int ei = (prow - A) / NPE, ej = (dsite - A) / NPE;
+      const int gi = ro.n2g[prow - A - NPE * ei], gj = ro.n2g[dsite - A - NPE * ej];
+      if (!gauss_pair_in_row(ro, L, ei, gi, ej, gj, T)) {
+        from = dsite;
+        to = prow;
+        sgn = -1;
+      }
+    }
+    for (int d = 0; d < 3; d++)
+      del[d] = x[(size_t)d * P + from] - (x[(size_t)d * P + to] + (double)(sgn * T[d]) * L[d]);
+  } else {
+    int e, u;
+    split_ucell(ug, -dsite - 1, e, u);
+    double N[NPE];
+    hex_shape_at(ug, u, N);
+    const size_t base = (size_t)A + (size_t)NPE * e;
+    for (int d = 0; d < 3; d++) {
+      double v = x[(size_t)d * P + prow];
+      const double sh = (double)T[d] * L[d];
+      for (int k = 0; k < NPE; k++) v -= N[k] * (x[(size_t)d * P + base + k] + sh);
+      del[d] = v;
+    }
+  }
+  return del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+}
+
 // BL lanes per row scan the 5x5x5 block of cells around the row's cell (cell
 // edge > (rc+skin)/2, periodic) and keep every site image within rc + skin
 // except the row's own site.  Sites are sorted by cell, so the five cells of
@@ -320,7 +412,9 @@ constexpr int BL = 8;
 __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__ site, const int* __restrict__ row_site,
                                                      int R, Grid g, const int* __restrict__ cstart, double cutsq,
                                                      int* __restrict__ nnum, unsigned* __restrict__ nlist, int cap,
-                                                     ErrorFlags* flags) {
+                                                     ErrorFlags* flags, const double* __restrict__ x, int P, int A,
+                                                     UcellGrid ug, RefOrder ro, const int* __restrict__ sdesc,
+                                                     const int* __restrict__ simg) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x % BL;
   const int gbase = (threadIdx.x & 63) & ~(BL - 1);
@@ -329,6 +423,7 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
   const int k = row_site[r];
   const double4 pk = site[k];
   const double px = pk.x, py = pk.y, pz = pk.z;
+  const double near = NEAR_CUT_REL * cutsq;
   int ic[3];
   const double pp[3] = {px, py, pz};
   for (int d = 0; d < 3; d++) {
@@ -372,7 +467,9 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
             const double delx = px - (sj.x + shx);
             const double dely = py - (sj.y + shy);
             const double delz = pz - (sj.z + shz);
-            const double rsq = delx * delx + dely * dely + delz * delz;
+            double rsq = delx * delx + dely * dely + delz * delz;
+            if (fabs(rsq - cutsq) <= near)
+              rsq = reference_rsq(x, P, A, ug, ro, g.L, sdesc[k], simg[k], sdesc[j], simg[j], tcode);
             keep = rsq <= cutsq && !(j == k && tcode == 13u);
           }
           const unsigned long long m = __ballot(keep);
@@ -528,13 +625,18 @@ void build_neighbor(Ctx& c) {
     c.cap = std::min(c.one, ((int)(1.5 * est) + 32 + 15) / 16 * 16);
   }
   c.nnum.reserve(P);
+  RefOrder ro;
+  ro.ex = c.E ? c.ex.get() : nullptr;
+  ro.E = c.E;
+  for (int k = 0; k < 8; k++) ro.n2g[k] = c.n2g[k];
   for (int attempt = 0; attempt < 3; attempt++) {
     c.nlist.reserve((size_t)c.cap * P);
     CAC_HIP(hipMemsetAsync(&c.d_flags.get()->overflow, 0, sizeof(int) * 2, c.stream));
     k_build_lists<<<grid_for((long)BL * P, 256), 256, 0, c.stream>>>(c.site.get(), c.row_site.get(), P, g, c.cell_start.get(),
                                                           c.cutneighsq, c.nnum.get(),
                                                           reinterpret_cast<unsigned*>(c.nlist.get()), c.cap,
-                                                          c.d_flags.get());
+                                                          c.d_flags.get(), c.x.get(), P, c.A, ucell_grid(c), ro,
+                                                          c.sdesc.get(), c.simg.get());
     read_flags(c);
     if (!c.h_flags->overflow) break;
     int need = c.h_flags->maxcount;

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_live_compact(const int* __restrict__ fl
   }
 }
 
-static UcellGrid ucell_grid(const Ctx& c) {
+UcellGrid ucell_grid(const Ctx& c) {
   UcellGrid ug;
   for (int d = 0; d < 3; d++) ug.h[d] = c.ncell[d] > 1 ? 2.0 / (c.ncell[d] - 1) : 0.0;
   ug.ny = c.ncell[1] > 0 ? c.ncell[1] : 1;
@@ -551,6 +551,10 @@ struct EamForceArgs {
   double rdr, rdrho, rhomax;
   int nr, nrho;
   double escale_node, vscale_node;
+  // per-point energy and virial [7][P] (Pair::eatom / enode, vatom / vnode),
+  // or null: each point's F(rho) plus half of its pair energies, and half of
+  // its pair virials (pair.cpp:441-476, pair_eam.cpp:863-918, 979-984)
+  double* peratom;
 };
 
 // Force (and, with EV, energy and virial) of the rows of chunk ch of one
@@ -697,6 +701,16 @@ __device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArg
       erow[(size_t)a.R * 4 + r] = vs * v3;
       erow[(size_t)a.R * 5 + r] = vs * v4;
       erow[(size_t)a.R * 6 + r] = vs * v5;
+      if (q.peratom) {
+        double* pa = q.peratom + pnt;
+        pa[0] = phi + 0.5 * ephi;
+        pa[(size_t)P] = 0.5 * v0;
+        pa[2 * (size_t)P] = 0.5 * v1;
+        pa[3 * (size_t)P] = 0.5 * v2;
+        pa[4 * (size_t)P] = 0.5 * v3;
+        pa[5 * (size_t)P] = 0.5 * v4;
+        pa[6 * (size_t)P] = 0.5 * v5;
+      }
     }
   }
 }
@@ -814,6 +828,7 @@ __global__ __launch_bounds__(256) void k_embed_seg(const SegSites sg, const doub
 
 struct LjArgs {
   double lj1, lj2, lj3, lj4, escale_node, vscale_node;
+  double* peratom;  // per-point energy and virial [7][P], or null (as EamForceArgs)
 };
 
 template <int TPR, bool MIXED, bool EV>
@@ -927,12 +942,26 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
         erow[(size_t)a.R * 4 + r] = vs * v3;
         erow[(size_t)a.R * 5 + r] = vs * v4;
         erow[(size_t)a.R * 6 + r] = vs * v5;
+        if (q.peratom) {
+          double* pa = q.peratom + pnt;
+          pa[0] = 0.5 * ev;
+          pa[(size_t)P] = 0.5 * v0;
+          pa[2 * (size_t)P] = 0.5 * v1;
+          pa[3 * (size_t)P] = 0.5 * v2;
+          pa[4 * (size_t)P] = 0.5 * v3;
+          pa[5 * (size_t)P] = 0.5 * v4;
+          pa[6 * (size_t)P] = 0.5 * v5;
+        }
       }
     }
   }
 }
 
 // ------------------------------------------------------------------ orchestration
+
+// ENERGY_ATOM, VIRIAL_ATOM or VIRIAL_CENTROID (force.h:11-19; the centroid
+// virial of these pair styles is the two-body one, pair.cpp:280-281)
+static bool peratom_wanted(int eflag, int vflag) { return (eflag & 2) || (vflag & 12); }
 
 static PairArgs pair_args(Ctx& c, double cutsq, int rows_per_wg) {
   PairArgs a;
@@ -1014,6 +1043,7 @@ static void run_force(Ctx& c, int eflag, int vflag, Gate g) {
   q.nrho = c.nrho;
   q.escale_node = c.nodal_weight / 2.0;
   q.vscale_node = c.nodal_weight / 2.0;
+  q.peratom = peratom_wanted(eflag, vflag) ? c.peratom.get() : nullptr;
   ScopedTimer tm(c, "eam_force");
   if (ev) {
     auto kf = k_eam_force<TPR, MIXED, true, NT, U>;
@@ -1066,7 +1096,8 @@ static void run_lj(Ctx& c, int eflag, int vflag, Gate g) {
   const int E8 = NPE * c.E;
   double* er = ev ? c.erow.get() : nullptr;
   PairArgs a = pair_args(c, c.ljcutsq, PB / TPR);
-  LjArgs q{c.lj1, c.lj2, c.lj3, c.lj4, c.nodal_weight / 2.0, c.nodal_weight / 2.0};
+  LjArgs q{c.lj1, c.lj2, c.lj3, c.lj4, c.nodal_weight / 2.0, c.nodal_weight / 2.0,
+           peratom_wanted(eflag, vflag) ? c.peratom.get() : nullptr};
   ScopedTimer tm(c, "lj_force");
   auto kl = mixed ? (ev ? k_lj_force<TPR, true, true> : k_lj_force<TPR, true, false>)
                   : (ev ? k_lj_force<TPR, false, true> : k_lj_force<TPR, false, false>);
@@ -1111,6 +1142,8 @@ void ensure_inner(Ctx& c) {
 void pair_passes(Ctx& c, int eflag, int vflag, Gate g) {
   const bool ev = eflag || vflag;
   if (ev) c.erow.reserve((size_t)7 * c.P);
+  if (peratom_wanted(eflag, vflag)) c.peratom.reserve((size_t)7 * c.P);
+  c.peratom_flags = ((eflag & 2) ? 1 : 0) | ((vflag & 12) ? 2 : 0);
   ensure_inner(c);
   launch_pair(c, eflag, vflag, g);
 }
@@ -1178,12 +1211,15 @@ void pair_passes_segmented(Ctx* const* cs, int n, hipStream_t stream) {
   CAC_HIP(hipGetLastError());
 }
 
+// eng_vdwl with ENERGY_GLOBAL, virial[6] with VIRIAL_PAIR or VIRIAL_FDOTR
+// (the pair styles here never form the f.r virial: no_virial_fdotr_compute,
+// pair.cpp:46, 276-279); per-point tallies alone leave both as they were
 void pair_sums(Ctx& c, int eflag, int vflag) {
-  if (!eflag && !vflag) return;
+  if (!(eflag & 1) && !(vflag & 3)) return;
   double out[7];
   reduce_rows(c, c.erow.get(), c.P, 7, out);
-  if (eflag) c.eng_vdwl = out[0];
-  if (vflag)
+  if (eflag & 1) c.eng_vdwl = out[0];
+  if (vflag & 3)
     for (int i = 0; i < 6; i++) c.virial[i] = out[1 + i];
 }
 

@@ -213,6 +213,19 @@ void Device::download_forces() {
 void Pair::compute(int eflag, int vflag) {
   dev_.check(cac_hip_compute(dev_.ctx(), eflag, vflag));
   dev_.check(cac_hip_get_energy(dev_.ctx(), &eng_vdwl, virial));
+  const System& s = dev_.system();
+  const bool ea = eflag & 2, va = vflag & 12;  // ENERGY_ATOM; VIRIAL_ATOM | VIRIAL_CENTROID
+  if (!ea && !va) return;
+  if (ea) {
+    eatom.assign((size_t)s.atom.n, 0.0);
+    enode.assign((size_t)s.element.n * 8, 0.0);
+  }
+  if (va) {
+    vatom.assign((size_t)s.atom.n * 6, 0.0);
+    vnode.assign((size_t)s.element.n * 48, 0.0);
+  }
+  dev_.check(cac_hip_get_peratom(dev_.ctx(), ea ? eatom.data() : nullptr, ea ? enode.data() : nullptr,
+                                 va ? vatom.data() : nullptr, va ? vnode.data() : nullptr));
 }
 
 void PairEAMAlloyHIP::settings(int narg, char**) {

@@ -93,10 +93,13 @@ class Pair {
   virtual void init_style() {}           // the neighbor request is implicit: full lists on device
   virtual double init_one(int i, int j) = 0;
   // Pair::compute + the reverse comm + Element::compute_nodef, on device;
-  // eflag / vflag as force.h:11-19; fills eng_vdwl and virial[6]
+  // eflag / vflag as force.h:11-19; fills eng_vdwl and virial[6], and with
+  // ENERGY_ATOM / VIRIAL_ATOM (or VIRIAL_CENTROID) the per-point tallies
   virtual void compute(int eflag, int vflag);
   double eng_vdwl = 0.0;
   double virial[6] = {0, 0, 0, 0, 0, 0};
+  // pair.h:13-17: eatom [natoms], enode [nelements][8], vatom [natoms][6], vnode [nelements][8][6]
+  std::vector<double> eatom, enode, vatom, vnode;
 
  protected:
   Device& dev_;

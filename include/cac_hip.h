@@ -89,10 +89,24 @@ int cac_hip_neigh_export(cac_hip_ctx *ctx, long *n, int *row, int *kind, int *ow
 /* ---- force: replaces Verlet::force_clear + Pair::compute(eflag, vflag) + the
  * reverse comm + Element::compute_nodef (verlet.cpp:192-212, 322-344;
  * pair_eam.cpp:618-1032, pair_lj_cut.cpp:227-417, element.cpp:2178-2246).
- * eflag: ENERGY_GLOBAL = 1, ENERGY_ATOM = 2; vflag: VIRIAL_PAIR = 1, VIRIAL_FDOTR = 2 (force.h:11-19) */
+ * eflag (0..3): ENERGY_GLOBAL = 1 (eng_vdwl), ENERGY_ATOM = 2 (eatom, enode);
+ * vflag (0..15): VIRIAL_PAIR = 1 and VIRIAL_FDOTR = 2 (virial[6]: these pair
+ * styles never form the f.r virial, pair.cpp:46, 276-279), VIRIAL_ATOM = 4 and
+ * VIRIAL_CENTROID = 8 (vatom, vnode: the centroid virial of a two-body style
+ * is its per-atom virial, pair.cpp:57, 280-281) (force.h:11-19).  Other
+ * values return CAC_HIP_EARG. */
 int cac_hip_compute(cac_hip_ctx *ctx, int eflag, int vflag);
-/* Pair::eng_vdwl and Pair::virial[6] of the last compute with eflag / vflag */
+/* Pair::eng_vdwl and Pair::virial[6] of the last compute with ENERGY_GLOBAL /
+ * a global virial flag (a compute without them leaves them as they were) */
 int cac_hip_get_energy(cac_hip_ctx *ctx, double *eng_vdwl, double *virial);
+/* Pair::eatom [natoms], enode [nelements][1][8], vatom [natoms][6] and
+ * vnode [nelements][1][8][6] of the last compute (pair.h:13-17, ev_tally
+ * pair.cpp:441-476): an atom or node holds its own embedding energy plus half
+ * of each of its pair energies, and half of each of its pair virials;
+ * interpolated atoms hold none (pair_eam.cpp:863-918, 971-984).  Each pointer
+ * may be NULL; asking for energies or virials that the last compute's flags
+ * did not request returns CAC_HIP_ESTATE. */
+int cac_hip_get_peratom(cac_hip_ctx *ctx, double *eatom, double *enode, double *vatom, double *vnode);
 
 /* ---- integrate: replaces FixNVE::initial_integrate / final_integrate (fix_nve.cpp:62-217) ---- */
 int cac_hip_nve_initial(cac_hip_ctx *ctx, double dt);

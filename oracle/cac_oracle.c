@@ -121,9 +121,10 @@ struct Orc {
   int nodal_force_style; /* 0 CONSISTENT (default, element.cpp:60), 1 LUMPED, 2 NODEONLY */
 
   /* ev */
-  int eflag_global, eflag_atom, vflag_global, evflag;
+  int eflag_global, eflag_atom, vflag_global, vflag_atom, evflag;
   double eng_vdwl, virial[6];
   double *eatom, *enode;
+  double *vatom, *vnode; /* [atoms][6], [elements][8][6] */
 };
 
 static int fail(Orc *o, int code, const char *fmt, ...) {
@@ -193,6 +194,7 @@ void orc_destroy(Orc *o) {
   free(o->pool_j); free(o->pool_idx); free(o->elemneigh);
   free(o->frho_spline); free(o->rhor_spline); free(o->z2r_spline);
   free(o->atomrho); free(o->atomfp); free(o->noderho); free(o->eatom); free(o->enode);
+  free(o->vatom); free(o->vnode);
   free(o);
 }
 
@@ -384,6 +386,7 @@ static void grow_atoms(Orc *o, int n) {
   o->atomrho = (double *)xrealloc(o->atomrho, sizeof(double) * m);
   o->atomfp = (double *)xrealloc(o->atomfp, sizeof(double) * m);
   o->eatom = (double *)xrealloc(o->eatom, sizeof(double) * m);
+  o->vatom = (double *)xrealloc(o->vatom, sizeof(double) * 6 * m);
   o->namax = m;
 }
 
@@ -404,6 +407,7 @@ static void grow_elems(Orc *o, int n) {
   o->eimg = (int *)xrealloc(o->eimg, sizeof(int) * 3 * m);
   o->noderho = (double *)xrealloc(o->noderho, sizeof(double) * m * NPE);
   o->enode = (double *)xrealloc(o->enode, sizeof(double) * m * NPE);
+  o->vnode = (double *)xrealloc(o->vnode, sizeof(double) * m * NPE * 6);
   o->nemax = m;
 }
 
@@ -558,8 +562,12 @@ int orc_pair_eam_arrays(Orc *o, int nrho, double drho, int nr, double dr, double
   return ORC_OK;
 }
 
-/* PairEAMAlloy::read_file, pair_eam_alloy.cpp:100-196, grab pair_eam.cpp:292-304.
-   Only the first element of the file is used (one-element runs). */
+/* PairEAMAlloy::read_file (pair_eam_alloy.cpp:100-196) with grab
+   (pair_eam.cpp:292-304), the element map of PairEAMAlloy::coeff
+   (pair_eam_alloy.cpp:57-69) and the table choice of file2array
+   (pair_eam_alloy.cpp:202-315): the one atom type takes the tables of the
+   element named type2elem[0] (element 0 when type2elem is NULL): its F(rho),
+   its rho(r) and the z2r block (i, i) of the lower triangle stored i >= j. */
 static int grab(FILE *fp, int n, double *list) {
   char line[4096];
   int i = 0;
@@ -574,38 +582,51 @@ static int grab(FILE *fp, int n, double *list) {
 }
 
 int orc_pair_eam_setfl(Orc *o, const char *path, const char *const *type2elem) {
-  (void)type2elem;
   FILE *fp = fopen(path, "r");
   if (!fp) return fail(o, ORC_ERR_ARG, "Cannot open EAM potential file %s", path);
   char line[4096];
   for (int i = 0; i < 4; i++)
     if (!fgets(line, sizeof(line), fp)) { fclose(fp); return fail(o, ORC_ERR_ARG, "short setfl"); }
-  int nelem = atoi(line);
-  if (nelem < 1) { fclose(fp); return fail(o, ORC_ERR_ARG, "Incorrect element names in EAM potential file"); }
+  /* nelements and the element names (pair_eam_alloy.cpp:130-150) */
+  char names[64][64];
+  int nelem = atoi(line), nwords = 0;
+  for (char *w = strtok(line, " \t\n\r\f"); w; w = strtok(NULL, " \t\n\r\f")) {
+    if (nwords >= 1 && nwords <= 64) snprintf(names[nwords - 1], sizeof(names[0]), "%s", w);
+    nwords++;
+  }
+  if (nelem < 1 || nelem > 64 || nwords != nelem + 1) {
+    fclose(fp);
+    return fail(o, ORC_ERR_ARG, "Incorrect element names in EAM potential file");
+  }
+  int map = 0;
+  if (type2elem && type2elem[0]) {
+    for (map = 0; map < nelem; map++)
+      if (strcmp(type2elem[0], names[map]) == 0) break;
+    if (map == nelem) { fclose(fp); return fail(o, ORC_ERR_ARG, "No matching element in EAM potential file"); }
+  }
   if (!fgets(line, sizeof(line), fp)) { fclose(fp); return fail(o, ORC_ERR_ARG, "short setfl"); }
-  int nrho, nr;
-  double drho, dr, cut;
-  if (sscanf(line, "%d %lg %d %lg %lg", &nrho, &drho, &nr, &dr, &cut) != 5) {
+  int nrho = 0, nr = 0;
+  double drho = 0.0, dr = 0.0, cut = 0.0;
+  if (sscanf(line, "%d %lg %d %lg %lg", &nrho, &drho, &nr, &dr, &cut) != 5 || nrho <= 0 || nr <= 0 || dr <= 0.0) {
     fclose(fp);
     return fail(o, ORC_ERR_ARG, "Invalid EAM potential file");
   }
   double *frho = (double *)xrealloc(NULL, sizeof(double) * nrho);
   double *rhor = (double *)xrealloc(NULL, sizeof(double) * nr);
   double *z2r = (double *)xrealloc(NULL, sizeof(double) * nr);
+  double *skip = (double *)xrealloc(NULL, sizeof(double) * MAX(nrho, nr));
   int rc = 0;
-  if (!fgets(line, sizeof(line), fp)) rc = -1; /* Z mass a lattice */
-  if (!rc) rc = grab(fp, nrho, frho);
-  if (!rc) rc = grab(fp, nr, rhor);
-  /* skip further elements' tables, then first z2r (i = 0, j = 0) */
-  for (int e = 1; !rc && e < nelem; e++) {
-    double *skip = (double *)xrealloc(NULL, sizeof(double) * MAX(nrho, nr));
+  /* per element: "Z mass a lattice", F(rho)[nrho], rho(r)[nr] */
+  for (int e = 0; !rc && e < nelem; e++) {
     if (!fgets(line, sizeof(line), fp)) rc = -1;
-    if (!rc) rc = grab(fp, nrho, skip);
-    if (!rc) rc = grab(fp, nr, skip);
-    free(skip);
+    if (!rc) rc = grab(fp, nrho, e == map ? frho : skip);
+    if (!rc) rc = grab(fp, nr, e == map ? rhor : skip);
   }
-  if (!rc) rc = grab(fp, nr, z2r);
+  /* z2r blocks (i, j), j <= i, in row order; keep (map, map) */
+  for (int i = 0; !rc && i < nelem; i++)
+    for (int j = 0; !rc && j <= i; j++) rc = grab(fp, nr, (i == map && j == map) ? z2r : skip);
   fclose(fp);
+  free(skip);
   if (rc) {
     free(frho); free(rhor); free(z2r);
     return fail(o, ORC_ERR_ARG, "truncated setfl %s", path);
@@ -1429,7 +1450,10 @@ static void ev_setup(Orc *o, int eflag, int vflag) {
   o->eflag_global = eflag & 1;
   o->eflag_atom = eflag & 2;
   o->vflag_global = (vflag & 1) || (vflag & 2);
-  o->evflag = (eflag != 0) || o->vflag_global;
+  /* VIRIAL_ATOM, and VIRIAL_CENTROID for a style whose centroid stress is the
+     two-body one (CENTROID_SAME, pair.cpp:57, 280-281) */
+  o->vflag_atom = (vflag & 4) || (vflag & 8);
+  o->evflag = (eflag != 0) || o->vflag_global || o->vflag_atom;
   if (o->eflag_global) o->eng_vdwl = 0.0;
   if (o->vflag_global)
     for (int i = 0; i < 6; i++) o->virial[i] = 0.0;
@@ -1437,11 +1461,15 @@ static void ev_setup(Orc *o, int eflag, int vflag) {
     memset(o->eatom, 0, sizeof(double) * (o->nalocal + o->naghost));
     memset(o->enode, 0, sizeof(double) * NPE * (o->nelocal + o->neghost));
   }
+  if (o->vflag_atom) {
+    memset(o->vatom, 0, sizeof(double) * 6 * (o->nalocal + o->naghost));
+    memset(o->vnode, 0, sizeof(double) * 6 * NPE * (o->nelocal + o->neghost));
+  }
 }
 
 /* Pair::ev_tally, pair.cpp:410-476 */
-static void ev_tally(Orc *o, double escale, double vscale, double *ieptr, double *jeptr, double evdwl,
-                     double fpair, double dx, double dy, double dz) {
+static void ev_tally(Orc *o, double escale, double vscale, double *ieptr, double *jeptr, double *ivptr,
+                     double *jvptr, double evdwl, double fpair, double dx, double dy, double dz) {
   if (o->eflag_global || o->eflag_atom) {
     if (o->eflag_global) o->eng_vdwl += evdwl * escale;
     if (o->eflag_atom) {
@@ -1450,15 +1478,17 @@ static void ev_tally(Orc *o, double escale, double vscale, double *ieptr, double
       if (jeptr) *jeptr += h;
     }
   }
-  if (o->vflag_global) {
-    double v0 = dx * dx * fpair, v1 = dy * dy * fpair, v2 = dz * dz * fpair;
-    double v3 = dx * dy * fpair, v4 = dx * dz * fpair, v5 = dy * dz * fpair;
-    o->virial[0] += v0 * vscale;
-    o->virial[1] += v1 * vscale;
-    o->virial[2] += v2 * vscale;
-    o->virial[3] += v3 * vscale;
-    o->virial[4] += v4 * vscale;
-    o->virial[5] += v5 * vscale;
+  if (o->vflag_global || o->vflag_atom) {
+    double v[6] = {dx * dx * fpair, dy * dy * fpair, dz * dz * fpair,
+                   dx * dy * fpair, dx * dz * fpair, dy * dz * fpair};
+    if (o->vflag_global)
+      for (int k = 0; k < 6; k++) o->virial[k] += v[k] * vscale;
+    if (o->vflag_atom) {
+      if (ivptr)
+        for (int k = 0; k < 6; k++) ivptr[k] += 0.5 * v[k];
+      if (jvptr)
+        for (int k = 0; k < 6; k++) jvptr[k] += 0.5 * v[k];
+    }
   }
 }
 
@@ -1487,7 +1517,7 @@ static void lj_compute(Orc *o, int eflag, int vflag) {
   ev_setup(o, eflag, vflag);
   for (int ii = 0; ii < o->inum; ii++) {
     int i = o->ilist[ii], iidx = o->iindex[ii];
-    double iescale = 0.0, ivscale = 0.0, *ieptr = NULL, *fi;
+    double iescale = 0.0, ivscale = 0.0, *ieptr = NULL, *ivptr = NULL, *fi;
     double xtmp, ytmp, ztmp;
     int ict;
     if (iidx < 0) {
@@ -1497,6 +1527,7 @@ static void lj_compute(Orc *o, int eflag, int vflag) {
       if (o->eflag_global) iescale = 0.5;
       if (o->vflag_global) ivscale = 0.5;
       if (o->eflag_atom) ieptr = &o->eatom[i];
+      if (o->vflag_atom) ivptr = &o->vatom[6 * i];
     } else {
       const EType *t = &o->et[o->eet[i]];
       int ig = iidx, inode = t->g2n[ig];
@@ -1516,18 +1547,20 @@ static void lj_compute(Orc *o, int eflag, int vflag) {
         if (o->eflag_global) iescale = t->nodal_weight / 2.0;
         if (o->vflag_global) ivscale = t->nodal_weight / 2.0;
         if (o->eflag_atom) ieptr = &o->enode[i * NPE + inode];
+        if (o->vflag_atom) ivptr = &o->vnode[(i * NPE + inode) * 6];
       }
     }
     long base = o->firstneigh[ii];
     for (int jj = 0; jj < o->numneigh[ii]; jj++) {
       int j = o->pool_j[base + jj], jidx = o->pool_idx[base + jj];
-      double d[3], jescale = 0.0, jvscale = 0.0, *fj = NULL, *jeptr = NULL;
+      double d[3], jescale = 0.0, jvscale = 0.0, *fj = NULL, *jeptr = NULL, *jvptr = NULL;
       int jct;
       del_to(o, j, jidx, xtmp, ytmp, ztmp, d);
       if (jidx < 0) {
         jct = o->atype[j];
         fj = &o->af[3 * j];
         if (o->eflag_atom) jeptr = &o->eatom[j];
+        if (o->vflag_atom) jvptr = &o->vatom[6 * j];
         if (o->eflag_global) jescale = 0.5;
         if (o->vflag_global) jvscale = 0.5;
       } else {
@@ -1539,6 +1572,7 @@ static void lj_compute(Orc *o, int eflag, int vflag) {
           fj = &o->gaussf[(j * o->ngmax + jg) * 3];
           if (jnode >= 0) {
             if (o->eflag_atom) jeptr = &o->enode[j * NPE + jnode];
+            if (o->vflag_atom) jvptr = &o->vnode[(j * NPE + jnode) * 6];
             if (o->eflag_global) jescale = t->nodal_weight / 2.0;
             if (o->vflag_global) jvscale = t->nodal_weight / 2.0;
           }
@@ -1560,7 +1594,8 @@ static void lj_compute(Orc *o, int eflag, int vflag) {
         }
         double evdwl = 0.0;
         if (eflag) evdwl = r6inv * (o->lj3[ict][jct] * r6inv - o->lj4[ict][jct]) - o->ljoff[ict][jct];
-        if (o->evflag) ev_tally(o, iescale + jescale, ivscale + jvscale, ieptr, jeptr, evdwl, fpair, d[0], d[1], d[2]);
+        if (o->evflag)
+          ev_tally(o, iescale + jescale, ivscale + jvscale, ieptr, jeptr, ivptr, jvptr, evdwl, fpair, d[0], d[1], d[2]);
       }
     }
   }
@@ -1664,7 +1699,7 @@ static void eam_compute(Orc *o, int eflag, int vflag) {
   /* force pass, 853-1027 */
   for (int ii = 0; ii < o->inum; ii++) {
     int i = o->ilist[ii], iidx = o->iindex[ii];
-    double iescale = 0.0, ivscale = 0.0, *ieptr = NULL, *fi, fpi;
+    double iescale = 0.0, ivscale = 0.0, *ieptr = NULL, *ivptr = NULL, *fi, fpi;
     double xtmp, ytmp, ztmp;
     int ict;
     if (iidx < 0) {
@@ -1674,6 +1709,7 @@ static void eam_compute(Orc *o, int eflag, int vflag) {
       if (o->eflag_global) iescale = 0.5;
       if (o->vflag_global) ivscale = 0.5;
       if (o->eflag_atom) ieptr = &o->eatom[i];
+      if (o->vflag_atom) ivptr = &o->vatom[6 * i];
       fi = &o->af[3 * i];
     } else {
       const EType *t = &o->et[o->eet[i]];
@@ -1700,6 +1736,7 @@ static void eam_compute(Orc *o, int eflag, int vflag) {
         }
         if (o->vflag_global) ivscale = t->nodal_weight / 2.0;
         if (o->eflag_atom) ieptr = &o->enode[i * NPE + inode];
+        if (o->vflag_atom) ivptr = &o->vnode[(i * NPE + inode) * 6];
       }
       fpi = compute_fp(o, rhoi, &coeff, &p, ict);
       if (eflag && inode >= 0) {
@@ -1713,7 +1750,7 @@ static void eam_compute(Orc *o, int eflag, int vflag) {
     long base = o->firstneigh[ii];
     for (int jj = 0; jj < o->numneigh[ii]; jj++) {
       int j = o->pool_j[base + jj], jidx = o->pool_idx[base + jj];
-      double d[3], jescale = 0.0, jvscale = 0.0, *fj = NULL, *jeptr = NULL, fpj;
+      double d[3], jescale = 0.0, jvscale = 0.0, *fj = NULL, *jeptr = NULL, *jvptr = NULL, fpj;
       int jct;
       if (jidx < 0) {
         del_to(o, j, jidx, xtmp, ytmp, ztmp, d);
@@ -1721,6 +1758,7 @@ static void eam_compute(Orc *o, int eflag, int vflag) {
         jct = o->atype[j];
         fj = &o->af[3 * j];
         if (o->eflag_atom) jeptr = &o->eatom[j];
+        if (o->vflag_atom) jvptr = &o->vatom[6 * j];
         if (o->eflag_global) jescale = 0.5;
         if (o->vflag_global) jvscale = 0.5;
       } else {
@@ -1743,6 +1781,7 @@ static void eam_compute(Orc *o, int eflag, int vflag) {
           fj = &o->gaussf[(j * o->ngmax + jg) * 3];
           if (jnode >= 0) {
             if (o->eflag_atom) jeptr = &o->enode[j * NPE + jnode];
+            if (o->vflag_atom) jvptr = &o->vnode[(j * NPE + jnode) * 6];
             if (o->eflag_global) {
               if (o->nodal_energy_weight < 0) jescale = t->nodal_weight / 2.0;
               else jescale = o->nodal_energy_weight / 2.0;
@@ -1781,7 +1820,8 @@ static void eam_compute(Orc *o, int eflag, int vflag) {
         }
         double evdwl = 0.0;
         if (eflag) evdwl = o->scale[ict][jct] * phi;
-        if (o->evflag) ev_tally(o, iescale + jescale, ivscale + jvscale, ieptr, jeptr, evdwl, fpair, d[0], d[1], d[2]);
+        if (o->evflag)
+          ev_tally(o, iescale + jescale, ivscale + jvscale, ieptr, jeptr, ivptr, jvptr, evdwl, fpair, d[0], d[1], d[2]);
       }
     }
   }
@@ -2107,6 +2147,23 @@ void orc_get_peratom_energy(Orc *o, double *eatom, double *enode) {
   if (enode) memcpy(enode, en, sizeof(double) * NPE * o->nelocal);
   free(ea);
   free(en);
+}
+
+void orc_get_peratom_virial(Orc *o, double *vatom, double *vnode) {
+  /* ghost per-atom virials are summed into owners (reverse comm of vatom) */
+  int na = o->nalocal + o->naghost, ne = o->nelocal + o->neghost;
+  double *va = (double *)xrealloc(NULL, sizeof(double) * 6 * MAX(na, 1));
+  double *vn = (double *)xrealloc(NULL, sizeof(double) * 6 * NPE * MAX(ne, 1));
+  memcpy(va, o->vatom, sizeof(double) * 6 * na);
+  memcpy(vn, o->vnode, sizeof(double) * 6 * NPE * ne);
+  for (int g = o->nalocal; g < na; g++)
+    for (int k = 0; k < 6; k++) va[6 * o->aown[g] + k] += va[6 * g + k];
+  for (int g = o->nelocal; g < ne; g++)
+    for (int q = 0; q < 6 * NPE; q++) vn[o->eown[g] * NPE * 6 + q] += vn[g * NPE * 6 + q];
+  if (vatom) memcpy(vatom, va, sizeof(double) * 6 * o->nalocal);
+  if (vnode) memcpy(vnode, vn, sizeof(double) * 6 * NPE * o->nelocal);
+  free(va);
+  free(vn);
 }
 
 int orc_neigh_stats(Orc *o, int *ncalls, int *ndanger) {

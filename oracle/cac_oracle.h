@@ -30,8 +30,8 @@
  * Scope of the restatement: orthogonal fully periodic box, one processor,
  * newton on, Hex elements with one basis atom per unit cell (apc = 1),
  * Gauss cells at the element corners (2x2x2, UNIFORM weight scale), the
- * CONSISTENT nodal force style.  These are the settings every benchmark
- * configuration uses.
+ * CONSISTENT (default), LUMPED and NODEONLY nodal force styles.  These are
+ * the settings every benchmark configuration uses.
  */
 #ifndef CAC_ORACLE_H
 #define CAC_ORACLE_H
@@ -92,6 +92,9 @@ void orc_get_elements(Orc *o, double *nodex, double *nodev, double *nodef, doubl
 void orc_get_rho(Orc *o, double *atomrho, double *noderho /* [n][8] */);
 void orc_get_energy(Orc *o, double *eng, double *virial /* [6] */);
 void orc_get_peratom_energy(Orc *o, double *eatom, double *enode /* [n][8] */);
+/* per-atom / per-node virial of the last compute with vflag & (VIRIAL_ATOM | VIRIAL_CENTROID):
+ * [natoms][6], [nelements][8][6] (Pair::vatom, vnode; pair.cpp:441-476) */
+void orc_get_peratom_virial(Orc *o, double *vatom, double *vnode);
 void orc_thermo(Orc *o, double *pe, double *ke, double *temp, double *press);
 int  orc_neigh_stats(Orc *o, int *ncalls, int *ndanger);
 /* element type tables for closed-form checks */

@@ -66,6 +66,7 @@ def lib():
         L.orc_get_rho.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.orc_get_energy.argtypes = [vp, C.POINTER(C.c_double), _dp]
         L.orc_get_peratom_energy.argtypes = [vp, C.c_void_p, C.c_void_p]
+        L.orc_get_peratom_virial.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.orc_thermo.argtypes = [vp] + [C.POINTER(C.c_double)] * 4
         L.orc_neigh_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_etype_tables.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
@@ -190,6 +191,11 @@ class Oracle:
         self.L.orc_get_peratom_energy(self.h, _ptr(ea), _ptr(en))
         return ea, en
 
+    def peratom_virial(self):
+        va, vn = np.zeros((self.deck.natoms, 6)), np.zeros((self.deck.nelements, 8, 6))
+        self.L.orc_get_peratom_virial(self.h, _ptr(va), _ptr(vn))
+        return va, vn
+
     def thermo(self):
         v = [C.c_double() for _ in range(4)]
         self.L.orc_thermo(self.h, *[C.byref(x) for x in v])
@@ -215,6 +221,26 @@ class Oracle:
         fs, rs, zs = np.zeros((a.value + 1, 7)), np.zeros((b.value + 1, 7)), np.zeros((b.value + 1, 7))
         self.L.orc_eam_splines(self.h, None, None, _ptr(fs), _ptr(rs), _ptr(zs))
         return fs, rs, zs
+
+
+def setfl_splines(path, element=None):
+    """Read a setfl file with the oracle's reader for one atom type mapped to
+    `element` (None: the file's first element) and return the spline tables it
+    built: F(rho), rho(r), z2r, each [n+1][7]."""
+    L = lib()
+    h = L.orc_create(1)
+    try:
+        names = (C.c_char_p * 1)(element.encode()) if element is not None else None
+        rc = L.orc_pair_eam_setfl(h, os.fsencode(path), names)
+        if rc:
+            raise OracleError(L.orc_errmsg(h).decode())
+        a, b = C.c_int(), C.c_int()
+        L.orc_eam_splines(h, C.byref(a), C.byref(b), None, None, None)
+        fs, rs, zs = np.zeros((a.value + 1, 7)), np.zeros((b.value + 1, 7)), np.zeros((b.value + 1, 7))
+        L.orc_eam_splines(h, None, None, _ptr(fs), _ptr(rs), _ptr(zs))
+        return fs, rs, zs
+    finally:
+        L.orc_destroy(h)
 
 
 def etype_tables(ncell=(9, 9, 9)):

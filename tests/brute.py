@@ -138,7 +138,7 @@ def _embed(fs, rho, tab):
     return np.where(rho > rhomax, F + Fp * (rho - rhomax), F), Fp
 
 
-def cac_eam(deck, tab, images=1):
+def cac_eam(deck, tab, images=1, peratom=False):
     """Composite CAC EAM by all-pairs enumeration, from the definitions alone:
     the density of every point (atom or Gauss point) sums f(r) over every site
     image within the cutoff; an interpolated atom's density is the
@@ -149,7 +149,9 @@ def cac_eam(deck, tab, images=1):
     pair energies.  No lists, bins, ghosts or half-list bookkeeping.
 
     Returns atom forces [A,3], Gauss forces [E,8 Gauss cells,3], atom
-    densities [A], nodal densities [E,8 nodes], energy."""
+    densities [A], nodal densities [E,8 nodes], energy; with peratom, also
+    each point's own energy F(rho) + 1/2 sum phi and virial 1/2 sum d d fpair
+    (atoms [A] / [A,6], nodes [E,8] / [E,8,6] in node order)."""
     pos, kind, owner, ucell = all_sites(deck)
     L = deck.boxhi - deck.boxlo
     A, E = deck.natoms, deck.nelements
@@ -191,6 +193,7 @@ def cac_eam(deck, tab, images=1):
     Fi, fp_i = _embed(fs, rho, tab)
     f = np.zeros((P, 3))
     phisum = np.zeros(P)
+    vir = np.zeros((P, 6))
     for p, (idx, d, r) in enumerate(pr):
         m, q = _knot_r(r, tab)
         _, rhop = spline_eval(rs, m, q)
@@ -200,6 +203,16 @@ def cac_eam(deck, tab, images=1):
         fpair = -((fp_i[p] + site_fp[idx]) * rhop + phip) / r
         f[p] = (d * fpair[:, None]).sum(0)
         phisum[p] = phi.sum()
+        vir[p] = 0.5 * np.stack([d[:, 0] * d[:, 0], d[:, 1] * d[:, 1], d[:, 2] * d[:, 2], d[:, 0] * d[:, 1],
+                                 d[:, 0] * d[:, 2], d[:, 1] * d[:, 2]], 1).T @ fpair
     nw = nu / 8.0
     energy = float(np.sum(Fi[:A] + 0.5 * phisum[:A]) + nw * np.sum(Fi[A:] + 0.5 * phisum[A:]))
+    if peratom:
+        ep = Fi + 0.5 * phisum
+        enode, vnode = np.zeros((E, 8)), np.zeros((E, 8, 6))
+        if E:
+            enode[:, g2n] = ep[A:].reshape(E, 8)
+            vnode[:, g2n, :] = vir[A:].reshape(E, 8, 6)
+        return f[:A], f[A:].reshape(E, 8, 3), rho[:A], rho_nodes, energy, dict(
+            eatom=ep[:A], enode=enode, vatom=vir[:A], vnode=vnode)
     return f[:A], f[A:].reshape(E, 8, 3), rho[:A], rho_nodes, energy

@@ -428,3 +428,192 @@ def test_periodic_wrap_moves_elements_rigidly():
     c = nb.mean(axis=1)
     k = np.floor((c - d.boxlo) / L)
     assert np.allclose(el["nodex"], nb - k[:, None, :] * L, atol=1e-9)
+
+
+# ---------------------------------------------------------------- setfl reader (pair_eam_alloy.cpp:29-315)
+
+# A two-element setfl written by hand: every table has distinct values and
+# tokens are split across lines irregularly, as grab (pair_eam.cpp:292-304)
+# allows.  Blocks: Cu F(rho), Cu rho(r), Ni F(rho), Ni rho(r), then z2r for
+# (Cu,Cu), (Ni,Cu), (Ni,Ni) (lower triangle, row order).
+SETFL_NRHO, SETFL_DRHO, SETFL_NR, SETFL_DR, SETFL_CUT = 6, 0.5, 7, 0.25, 1.5
+SETFL_TABLES = {
+    "Cu": ([0.0, -1.0, -1.5, -1.75, -1.9, -2.0], [3.0, 2.5, 2.0, 1.4, 0.9, 0.4, 0.0]),
+    "Ni": ([0.0, -1.2, -1.7, -2.1, -2.3, -2.4], [4.0, 3.1, 2.2, 1.5, 0.8, 0.3, 0.1]),
+}
+SETFL_Z2R = {("Cu", "Cu"): [9.0, 5.0, 2.0, 0.5, -0.5, -0.2, 0.0],
+             ("Ni", "Cu"): [8.0, 4.0, 1.5, 0.25, -0.6, -0.3, 0.0],
+             ("Ni", "Ni"): [7.0, 3.5, 1.0, 0.0, -0.7, -0.4, -0.1]}
+
+
+def _write_setfl2(path, names_line="2 Cu Ni", truncate=0):
+    def lines_of(vals, per):
+        return [" ".join(repr(v) for v in vals[i:i + per]) for i in range(0, len(vals), per)]
+
+    out = ["comment 1", "comment 2", "comment 3", names_line,
+           f"{SETFL_NRHO} {SETFL_DRHO} {SETFL_NR} {SETFL_DR} {SETFL_CUT}"]
+    for el, per in (("Cu", 4), ("Ni", 5)):
+        out.append(f"{29 if el == 'Cu' else 28} {63.546 if el == 'Cu' else 58.69} 3.615 fcc")
+        frho, rhor = SETFL_TABLES[el]
+        out += lines_of(frho, per) + lines_of(rhor, per + 1)
+    for key in (("Cu", "Cu"), ("Ni", "Cu"), ("Ni", "Ni")):
+        out += lines_of(SETFL_Z2R[key], 3)
+    text = "\n".join(out) + "\n"
+    if truncate:
+        text = text[:-truncate]
+    path.write_text(text)
+
+
+@pytest.mark.parametrize("element", ["Cu", "Ni"])
+def test_setfl_reader_picks_the_mapped_element(tmp_path, element):
+    """The element an atom type maps to supplies F(rho), rho(r) and the z2r
+    block (i, i) of the lower triangle; reading any other block (the other
+    element, or the mixed (Ni, Cu) pair) gives different splines."""
+    path = tmp_path / "two.setfl"
+    _write_setfl2(path)
+    fs, rs, zs = O.setfl_splines(str(path), element)
+
+    def spl(vals, delta):
+        return O.interpolate(np.concatenate([[0.0], vals]), delta)
+
+    frho, rhor = SETFL_TABLES[element]
+    assert np.array_equal(fs, spl(frho, SETFL_DRHO))
+    assert np.array_equal(rs, spl(rhor, SETFL_DR))
+    assert np.array_equal(zs, spl(SETFL_Z2R[(element, element)], SETFL_DR))
+    other = "Ni" if element == "Cu" else "Cu"
+    assert not np.array_equal(fs, spl(SETFL_TABLES[other][0], SETFL_DRHO))
+    assert not np.array_equal(zs, spl(SETFL_Z2R[("Ni", "Cu")], SETFL_DR))
+    # the 7th coefficient of each knot is the tabulated value itself (1-based knots)
+    assert np.array_equal(zs[1:, 6], SETFL_Z2R[(element, element)])
+
+
+def test_setfl_reader_errors(tmp_path):
+    """The reference's errors: an unknown element, a names line whose word
+    count is not nelements + 1, a short file."""
+    path = tmp_path / "two.setfl"
+    _write_setfl2(path)
+    with pytest.raises(O.OracleError, match="No matching element"):
+        O.setfl_splines(str(path), "Fe")
+    _write_setfl2(path, names_line="2 Cu")
+    with pytest.raises(O.OracleError, match="Incorrect element names"):
+        O.setfl_splines(str(path), "Cu")
+    _write_setfl2(path, truncate=8)
+    with pytest.raises(O.OracleError, match="truncated"):
+        O.setfl_splines(str(path), "Ni")
+
+
+def test_setfl_round_trip_of_the_benchmark_potential(tmp_path):
+    """The one-element file decks.write_setfl writes for the benchmark
+    potential reads back into the same splines as the arrays themselves."""
+    path = tmp_path / "cu.setfl"
+    decks.write_setfl(str(path), TAB)
+    fs, rs, zs = O.setfl_splines(str(path))
+    o = O.Oracle(decks.lj_block(n=2), "eam", skin=1.0, eam_tables=TAB)
+    fs2, rs2, zs2 = o.eam_splines()
+    assert np.array_equal(fs, fs2[: fs.shape[0]]) and np.array_equal(rs, rs2) and np.array_equal(zs, zs2)
+
+
+# ---------------------------------------------------------------- nodal force styles (element.cpp:2178-2246)
+
+def _nodef_style(style, nc=5):
+    d = decks.cac_eam(box_cells=(2, 2, 2), core_cells=4, ncell=nc, jitter=0.05, T=0.0, seed=31)
+    o = O.Oracle(d, "eam", skin=1.0, eam_tables=TAB, nodal_force_style=style)
+    o.setup()
+    return o.elements()
+
+
+def _node_gets_its_gauss_force(el, n2g):
+    """nodef[k] == gaussf[n2g[k]] exactly, for every element and node."""
+    return np.array_equal(el["nodef"], el["gaussf"][:, n2g, :])
+
+
+@pytest.mark.parametrize("style", [1, 2])
+def test_lumped_and_nodeonly_closed_form(style):
+    """2x2x2 Gauss cells at the corners with the UNIFORM weight scale: the
+    LUMPED weights N(g) w_g / nodal_weight are delta(g, n2g[k]) exactly
+    (element_vec_cac.cpp:3255-3260), so LUMPED, like NODEONLY
+    (element.cpp:2229-2240), gives each node its own Gauss cell's force."""
+    t = O.etype_tables((5, 5, 5))
+    el = _nodef_style(style)
+    assert np.abs(el["gaussf"]).max() > 1e-3
+    assert _node_gets_its_gauss_force(el, t["n2g"])
+
+
+def test_lumped_check_catches_consistent_weights():
+    """The closed-form check above rejects the plausible mistakes: the
+    CONSISTENT weights N w / nucell in the LUMPED branch (nodef = gaussf / 8),
+    and the CONSISTENT style itself (Minv applied)."""
+    t = O.etype_tables((5, 5, 5))
+    el = _nodef_style(1)
+    planted = dict(el)
+    planted["nodef"] = np.einsum("gk,egd->ekd", t["wsa"], el["gaussf"])  # sum_g wsa[g][k] gaussf[g]
+    assert np.allclose(planted["nodef"], el["gaussf"][:, t["n2g"], :] / 8.0, rtol=1e-14, atol=0)
+    assert not _node_gets_its_gauss_force(planted, t["n2g"])
+    assert not _node_gets_its_gauss_force(_nodef_style(0), t["n2g"])
+
+
+# ---------------------------------------------------------------- per-atom tallies (pair.cpp:266-476)
+
+def _lj_peratom(x, L, cut=2.5):
+    """Per-atom LJ energy 1/2 sum phi and virial 1/2 sum d d fpair over all
+    minimum images within the cutoff (eps = sigma = 1)."""
+    d = x[:, None, :] - x[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = np.einsum("ijk,ijk->ij", d, d)
+    np.fill_diagonal(r2, np.inf)
+    m = r2 < cut * cut
+    r2i = np.where(m, 1.0 / np.where(m, r2, 1.0), 0.0)
+    r6 = r2i ** 3
+    e = 0.5 * np.sum(4.0 * (r6 * r6 - r6), axis=1)
+    fpair = r6 * (48.0 * r6 - 24.0) * r2i
+    comps = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    v = np.stack([0.5 * np.sum(d[..., a] * d[..., b] * fpair, axis=1) for a, b in comps], 1)
+    return e, v
+
+
+@pytest.mark.parametrize("vflag", [4, 8])
+def test_lj_peratom_tallies_match_all_pairs(vflag):
+    """ENERGY_ATOM and VIRIAL_ATOM (or VIRIAL_CENTROID, which falls back to the
+    two-body per-atom virial for pair styles with CENTROID_SAME) give each atom
+    half of each of its pair energies and pair virials (pair.cpp:280-281, 441-476)."""
+    d = _lj_deck()
+    o = O.Oracle(d, "lj", skin=0.3, delay=0)
+    o.setup(eflag=3, vflag=1 | vflag)
+    x, _, _ = o.atoms()
+    e_b, v_b = _lj_peratom(x, d.boxhi - d.boxlo)
+    ea, _ = o.peratom_energy()
+    va, _ = o.peratom_virial()
+    assert np.allclose(ea, e_b, rtol=1e-12, atol=1e-13)
+    assert np.abs(va - v_b).max() < 1e-11 * np.abs(v_b).max()
+    e, vir = o.energy()
+    assert ea.sum() == pytest.approx(e, rel=1e-12)
+    assert np.allclose(va.sum(0), vir, rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("nc,box,core", [(3, (3, 3, 3), 8), (3, (3, 3, 3), 0), (5, (2, 2, 2), 4)])
+def test_cac_eam_peratom_tallies_match_definition(nc, box, core):
+    """Per-point energies and virials of the half list with reactions equal the
+    definitions: an atom or node gets F(rho) plus half of each of its pair
+    energies, and half of each pair virial (interpolated non-Gauss sites get
+    nothing, pair_eam.cpp:863-918, 971-984); weighting nodes by nodal_weight
+    the sums are the global tallies."""
+    dk = decks.cac_eam(box_cells=box, core_cells=core, ncell=nc, jitter=0.06, T=0.0, seed=41 + nc)
+    o = O.Oracle(dk, "eam", skin=1.0, eam_tables=TAB)
+    o.setup(eflag=3, vflag=5)
+    *_, e_b, pa = brute.cac_eam(dk, TAB, peratom=True)
+    ea, en = o.peratom_energy()
+    va, vn = o.peratom_virial()
+    escale = max(np.abs(pa["enode"]).max(), np.abs(pa["eatom"]).max() if dk.natoms else 0.0)
+    vscale = max(np.abs(pa["vnode"]).max(), np.abs(pa["vatom"]).max() if dk.natoms else 0.0)
+    if dk.natoms:
+        assert np.abs(ea - pa["eatom"]).max() < 1e-12 * escale
+        assert np.abs(va - pa["vatom"]).max() < 1e-10 * vscale
+    assert np.abs(en - pa["enode"]).max() < 1e-12 * escale
+    assert np.abs(vn - pa["vnode"]).max() < 1e-10 * vscale
+    nw = nc ** 3 / 8.0
+    e, vir = o.energy()
+    assert ea.sum() + nw * en.sum() == pytest.approx(e, rel=1e-12)
+    assert np.allclose(va.sum(0) + nw * vn.sum((0, 1)), vir, rtol=1e-10, atol=1e-9)
+    # a planted mistake: tallying the nodal-weighted share per node instead of
+    # the half misses the definition
+    assert np.abs(nw * en - pa["enode"]).max() > 1e-3 * escale
