@@ -94,6 +94,7 @@ def lib():
             "cac_hip_setup": [vp, i, i],
             "cac_hip_run": [vp, i, d, i, vp],
             "cac_hip_thermo": [vp, C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(d)],
+            "cac_hip_thermo_noalloc": [vp, C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(d)],
             "cac_hip_download_atoms": [vp, vp, vp, vp],
             "cac_hip_download_elements": [vp, vp, vp, vp, vp],
             "cac_hip_download_rho": [vp, vp, vp],
@@ -247,9 +248,12 @@ class Model:
         self._chk(self.L.cac_hip_get_peratom(self.h, _p(ea), _p(en), _p(va), _p(vn)))
         return dict(eatom=ea, enode=en, vatom=va, vnode=vn)
 
-    def thermo(self):
+    def thermo(self, noalloc=False):
+        """PE, KE, temperature, pressure; noalloc: through the entry point
+        that makes no device allocation."""
         v = [C.c_double() for _ in range(4)]
-        self._chk(self.L.cac_hip_thermo(self.h, *[C.byref(x) for x in v]))
+        fn = self.L.cac_hip_thermo_noalloc if noalloc else self.L.cac_hip_thermo
+        self._chk(fn(self.h, *[C.byref(x) for x in v]))
         return dict(zip(["pe", "ke", "temp", "press"], [x.value for x in v]))
 
     def neigh_info(self):

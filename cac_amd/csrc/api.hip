@@ -391,6 +391,7 @@ int cac_hip_create(cac_hip_ctx** out, int device, int units) {
     c.device = device;
     CAC_HIP(hipSetDevice(device));
     CAC_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.scratch = scratch_on(device);
     CAC_HIP(hipHostMalloc((void**)&c.h_red, sizeof(double) * 64, hipHostMallocDefault));
     CAC_HIP(hipHostMalloc((void**)&c.h_flags, sizeof(ErrorFlags), hipHostMallocDefault));
     c.d_flags.reserve(1);
@@ -905,6 +906,14 @@ int cac_hip_setup(cac_hip_ctx* h, int eflag, int vflag) {
   });
 }
 
+// ComputePressure::compute_scalar (compute_pressure.cpp:140-180) from the
+// temperature and the pair virial of the last global tally
+static double pressure(const Ctx& c, double t) {
+  const double dof = 3.0 * ((double)c.A + (double)c.nucell * c.E);
+  const double inv_volume = 1.0 / (c.L[0] * c.L[1] * c.L[2]);
+  return (dof * c.boltz * t + c.virial[0] + c.virial[1] + c.virial[2]) / 3.0 * inv_volume * c.nktv2p;
+}
+
 int cac_hip_thermo(cac_hip_ctx* h, double* pe, double* ke, double* temp, double* press) {
   return guarded(h, [&](Ctx& c) -> int {
     double k = 0, t = 0;
@@ -912,11 +921,21 @@ int cac_hip_thermo(cac_hip_ctx* h, double* pe, double* ke, double* temp, double*
     if (pe) *pe = c.eng_vdwl;
     if (ke) *ke = k;
     if (temp) *temp = t;
-    if (press) {
-      const double dof = 3.0 * ((double)c.A + (double)c.nucell * c.E);
-      const double inv_volume = 1.0 / (c.L[0] * c.L[1] * c.L[2]);
-      *press = (dof * c.boltz * t + c.virial[0] + c.virial[1] + c.virial[2]) / 3.0 * inv_volume * c.nktv2p;
-    }
+    if (press) *press = pressure(c, t);
+    return CAC_HIP_OK;
+  });
+}
+
+int cac_hip_thermo_noalloc(cac_hip_ctx* h, double* pe, double* ke, double* temp, double* press) {
+  return guarded(h, [&](Ctx& c) -> int {
+    if (!c.model_set) return fail(h, CAC_HIP_ESTATE, "no atoms or elements uploaded");
+    CAC_HIP(hipSetDevice(c.device));
+    double k = 0, t = 0;
+    thermo_noalloc(c, &k, &t);
+    if (pe) *pe = c.eng_vdwl;
+    if (ke) *ke = k;
+    if (temp) *temp = t;
+    if (press) *press = pressure(c, t);
     return CAC_HIP_OK;
   });
 }
@@ -930,9 +949,7 @@ int cac_hip_run(cac_hip_ctx* h, int nsteps, double dt, int thermo_every, double*
     auto record = [&]() {
       double k = 0, t = 0;
       thermo(c, &k, &t);
-      const double dof = 3.0 * ((double)c.A + (double)c.nucell * c.E);
-      const double inv_volume = 1.0 / (c.L[0] * c.L[1] * c.L[2]);
-      const double press = (dof * c.boltz * t + c.virial[0] + c.virial[1] + c.virial[2]) / 3.0 * inv_volume * c.nktv2p;
+      const double press = pressure(c, t);
       double* r = th + 5 * nt++;
       r[0] = (double)c.ntimestep;
       r[1] = c.eng_vdwl;

@@ -299,6 +299,7 @@ struct Ctx {
   double* h_stage = nullptr;        // pinned staging of the per-step host handover
   size_t h_stage_n = 0;
   DBuf<ErrorFlags> d_flags;
+  double* scratch = nullptr;  // device address of the allocation-free scratch array on this device
 
   // timing: an event pair on the context stream around each timed section,
   // read back only when the times are asked for (or the pool is full), so a
@@ -379,6 +380,11 @@ void step_end_begin(Ctx& c, double dt, int next_step, bool outer);
 void update_centers(Ctx& c);
 void nodal_assembly(Ctx& c);
 void thermo(Ctx& c, double* ke, double* temp);
+// thermo() with no device allocation: the sums go through the file-scope
+// scratch array of integrate.hip (c.scratch, its address on c's device)
+void thermo_noalloc(Ctx& c, double* ke, double* temp);
+// that array's device address on `device` (current), taken once per device
+double* scratch_on(int device);
 void reduce_rows(Ctx& c, const double* data, int n, int ncomp, double* out);
 
 // timing helpers (api.hip)

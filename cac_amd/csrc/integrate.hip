@@ -5,6 +5,9 @@
 // ElementVecCAC::update_center_coord (element_vec_cac.cpp:3740-3765),
 // Element::compute_nodef (element.cpp:2178-2246) and the thermo sums of
 // ComputeTemp / ComputePE / ComputePressure (compute_temp.cpp:73-157).
+#include <map>
+#include <mutex>
+
 #include "cac_ctx.hpp"
 
 namespace cac {
@@ -451,15 +454,13 @@ struct KeTab {
   int consistent;
 };
 
-__global__ void k_ke_rows(const double* __restrict__ v, const int* __restrict__ ptype, int A, int E, int P, KeTab t,
-                          double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = A + E;
-  if (i >= n) return;
+// m v^2 of row i: atom i < A, or element i - A with its consistent mass
+// matrix times m nucell (or its nodes times m nodal_weight without one)
+__device__ __forceinline__ double ke_row(const double* __restrict__ v, const int* __restrict__ ptype, int A, int P,
+                                         const KeTab& t, int i) {
   if (i < A) {
     const double vx = v[i], vy = v[(size_t)P + i], vz = v[2 * (size_t)P + i];
-    out[i] = t.m[ptype[i]] * (vx * vx + vy * vy + vz * vz);
-    return;
+    return t.m[ptype[i]] * (vx * vx + vy * vy + vz * vz);
   }
   const int e = i - A;
   const size_t b = (size_t)A + (size_t)NPE * e;
@@ -469,16 +470,22 @@ __global__ void k_ke_rows(const double* __restrict__ v, const int* __restrict__ 
       for (int l = 0; l < NPE; l++)
         sum += t.mm[k][l] * (v[b + k] * v[b + l] + v[(size_t)P + b + k] * v[(size_t)P + b + l] +
                              v[2 * (size_t)P + b + k] * v[2 * (size_t)P + b + l]);
-    out[i] = sum * t.m[ptype[b]] * t.nucell;
-  } else {
-    for (int k = 0; k < NPE; k++)
-      sum += v[b + k] * v[b + k] + v[(size_t)P + b + k] * v[(size_t)P + b + k] +
-             v[2 * (size_t)P + b + k] * v[2 * (size_t)P + b + k];
-    out[i] = sum * t.m[ptype[b]] * t.nodal_weight;
+    return sum * t.m[ptype[b]] * t.nucell;
   }
+  for (int k = 0; k < NPE; k++)
+    sum += v[b + k] * v[b + k] + v[(size_t)P + b + k] * v[(size_t)P + b + k] +
+           v[2 * (size_t)P + b + k] * v[2 * (size_t)P + b + k];
+  return sum * t.m[ptype[b]] * t.nodal_weight;
 }
 
-void thermo(Ctx& c, double* ke, double* temp) {
+__global__ void k_ke_rows(const double* __restrict__ v, const int* __restrict__ ptype, int A, int E, int P, KeTab t,
+                          double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A + E) return;
+  out[i] = ke_row(v, ptype, A, P, t, i);
+}
+
+static KeTab ke_tab(const Ctx& c) {
   KeTab t;
   for (int i = 0; i <= MAXT; i++) t.m[i] = c.mass[i];
   for (int a = 0; a < 8; a++)
@@ -486,16 +493,94 @@ void thermo(Ctx& c, double* ke, double* temp) {
   t.nucell = c.nucell;
   t.nodal_weight = c.nodal_weight;
   t.consistent = c.nodef_style == 0;
-  const int n = c.A + c.E;
-  c.erow.reserve((size_t)7 * c.P);
-  k_ke_rows<<<grid_for(n, 256), 256, 0, c.stream>>>(c.v.get(), c.ptype.get(), c.A, c.E, c.P, t, c.erow.get());
-  double s = 0.0;
-  reduce_rows(c, c.erow.get(), n, 1, &s);
+  return t;
+}
+
+// temperature and kinetic energy from sum m v^2 (compute_temp.cpp:73-95, 140-157)
+static void ke_temp(const Ctx& c, double s, double* ke, double* temp) {
   const double dof = 3.0 * ((double)c.A + (double)c.nucell * c.E);
   const double tfactor = dof > 0 ? c.mvv2e / (dof * c.boltz) : 0.0;
   const double T = s * tfactor;
   if (temp) *temp = T;
   if (ke) *ke = T * 0.5 * dof * c.boltz;
+}
+
+void thermo(Ctx& c, double* ke, double* temp) {
+  const int n = c.A + c.E;
+  c.erow.reserve((size_t)7 * c.P);
+  k_ke_rows<<<grid_for(n, 256), 256, 0, c.stream>>>(c.v.get(), c.ptype.get(), c.A, c.E, c.P, ke_tab(c),
+                                                   c.erow.get());
+  double s = 0.0;
+  reduce_rows(c, c.erow.get(), n, 1, &s);
+  ke_temp(c, s, ke, temp);
+}
+
+// ------------------------------------------------------------------ thermo without device allocations
+// The kinetic-energy sum of thermo() without the per-row buffer and the block
+// partials the context grows on demand: each workgroup sums the m v^2 of a
+// fixed slice of KE_RPB rows in a fixed tree and writes one partial into
+// `scratch`, a file-scope device array; one workgroup then sums the partials
+// in order into the slot after them, which is copied to pinned host memory
+// the context allocated when it was created.  Every GPU holds its own copy of
+// the array until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returned on that device (scratch_on).  Calls
+// that overlap on one GPU share the array and race; each call writes every
+// partial it reads before it reads it.
+constexpr int KE_RPB = 256 * 16;           // rows per workgroup
+constexpr int KE_MAX_BLOCKS = 4096;        // partials the array holds
+constexpr long THERMO_NOALLOC_MAX_ROWS = (long)KE_RPB * KE_MAX_BLOCKS;  // 16,777,216 atoms + elements
+static __device__ __align__(256) double scratch[KE_MAX_BLOCKS + 64];
+
+__global__ __launch_bounds__(RB) void k_ke_partial(const double* __restrict__ v, const int* __restrict__ ptype, int A,
+                                                   int E, int P, KeTab t, double* __restrict__ partial) {
+  __shared__ double s[RB];
+  const long first = (long)blockIdx.x * KE_RPB;
+  const long last = first + KE_RPB < (long)A + E ? first + KE_RPB : (long)A + E;
+  double acc = 0.0;
+  for (long i = first + threadIdx.x; i < last; i += RB) acc += ke_row(v, ptype, A, P, t, (int)i);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = RB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// the device address of `scratch` on each device, taken once per device
+static std::mutex scratch_mu;
+static std::map<int, double*> scratch_addr;
+
+double* scratch_on(int device) {
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  auto it = scratch_addr.find(device);
+  if (it != scratch_addr.end()) return it->second;
+  int cur = 0;
+  CAC_HIP(hipGetDevice(&cur));
+  if (cur != device) throw HipError{hipErrorInvalidDevice, "scratch_on: the device must be current"};
+  double* ptr = nullptr;
+  CAC_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+  scratch_addr[device] = ptr;
+  return ptr;
+}
+
+void thermo_noalloc(Ctx& c, double* ke, double* temp) {
+  const long n = (long)c.A + c.E;
+  if (n > THERMO_NOALLOC_MAX_ROWS)
+    throw HipError{hipErrorInvalidValue, "thermo without allocation: more than 16,777,216 atoms + elements"};
+  double* part = c.scratch;
+  if (!part) throw HipError{hipErrorInvalidValue, "thermo without allocation: no scratch address on this device"};
+  double s = 0.0;
+  if (n > 0) {
+    const int nb = (int)((n + KE_RPB - 1) / KE_RPB);
+    k_ke_partial<<<nb, RB, 0, c.stream>>>(c.v.get(), c.ptype.get(), c.A, c.E, c.P, ke_tab(c), part);
+    k_reduce_final<<<1, RB, 0, c.stream>>>(part, nb, 1, part + nb);
+    CAC_HIP(hipGetLastError());
+    CAC_HIP(hipMemcpyAsync(c.h_red, part + nb, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CAC_HIP(hipStreamSynchronize(c.stream));
+    s = c.h_red[0];
+  }
+  ke_temp(c, s, ke, temp);
 }
 
 }  // namespace cac

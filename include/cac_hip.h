@@ -121,6 +121,13 @@ int cac_hip_run(cac_hip_ctx *ctx, int nsteps, double dt, int thermo_every, doubl
 /* ---- observables: ComputePE / ComputeTemp / ComputePressure scalars (compute_pe.cpp:46-66,
  * compute_temp.cpp:73-157, compute_pressure.cpp:140-180) */
 int cac_hip_thermo(cac_hip_ctx *ctx, double *pe, double *ke, double *temp, double *press);
+/* cac_hip_thermo without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under graph capture,
+ * tight serving loops).  The sums go through a fixed device array of the
+ * library (4,160 doubles per GPU, held until the process ends), so the model
+ * may hold at most 16,777,216 atoms + elements (larger: CAC_HIP_EARG).  Calls
+ * on one GPU share that array: calls that overlap in time on one GPU race. */
+int cac_hip_thermo_noalloc(cac_hip_ctx *ctx, double *pe, double *ke, double *temp, double *press);
 
 /* ---- per-step host handover, for a caller that keeps the integrator on the host:
  * Pair::compute reads atom->x and element->nodex and writes atom->f,
