@@ -42,7 +42,7 @@ class _Result(C.Structure):
     _fields_ = [("status", C.c_int), ("nsteps", C.c_int), ("atom_equivalents", C.c_long),
                 ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
                 ("step_ms_mean", C.c_double), ("pe", C.c_double), ("ke", C.c_double), ("etotal", C.c_double),
-                ("builds", C.c_long)]
+                ("builds", C.c_long), ("device_step_ms_mean", C.c_double)]
 
 
 class CacHipError(RuntimeError):
@@ -118,6 +118,8 @@ def lib():
             "cac_hip_server_stats": [vp, C.POINTER(d), C.POINTER(d), C.POINTER(C.c_long), C.POINTER(d),
                                      C.POINTER(i), C.POINTER(d)],
             "cac_hip_server_segments": [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)],
+            "cac_hip_server_device_intervals": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                C.POINTER(C.c_long)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -391,9 +393,12 @@ class Server:
                                               C.byref(peak), C.byref(hbm)))
         sl, ss = C.c_long(), C.c_long()
         self._chk(self.L.cac_hip_server_segments(self.h, C.byref(sl), C.byref(ss)))
+        d50, d99, dn = C.c_double(), C.c_double(), C.c_long()
+        self._chk(self.L.cac_hip_server_device_intervals(self.h, C.byref(d50), C.byref(d99), C.byref(dn)))
         return dict(p50_ms=p50.value, p99_ms=p99.value, steps=n.value, atom_steps_per_s=rate.value,
                     peak_active=peak.value, peak_hbm_bytes=hbm.value, segmented_launches=sl.value,
-                    segmented_steps=ss.value)
+                    segmented_steps=ss.value, device_p50_ms=d50.value, device_p99_ms=d99.value,
+                    device_steps=dn.value)
 
     def close(self):
         if getattr(self, "h", None):

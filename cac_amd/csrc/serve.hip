@@ -58,10 +58,23 @@ struct Outcome {
   bool done = false;
 };
 
+// Device-side step completion: an event recorded on the slot's stream after
+// each step of its replica (and after the load), read back when it has
+// completed; the time between two consecutive completions is the replica's
+// step interval as the device ran it, queueing behind the other replicas'
+// kernels included.  At most MARKS - 1 marks are outstanding, so the event of
+// the last completion read is never recorded over before it is used.
+constexpr int MARKS = 16;
+
 struct Slot {
   cac_hip_ctx* h = nullptr;
   double held = 0.0;  // device bytes this slot's buffers are estimated to hold
   bool busy = false;
+  hipEvent_t marks[MARKS] = {};
+  long head = 0, tail = 0;  // marks recorded / read back
+  int last = -1;            // the last completed mark of the current replica, or -1
+  double dev_sum = 0.0;     // the current replica's device step intervals so far (ms)
+  long dev_n = 0;
 };
 
 struct Active {
@@ -117,6 +130,7 @@ struct cac_hip_server {
 
   // statistics, written by the service thread under mu
   std::vector<float> interval_ms;
+  std::vector<float> device_ms;  // step completion intervals on the device
   long seg_launches = 0, seg_steps = 0;
   double peak_hbm = 0.0;
   int peak_active = 0;
@@ -132,6 +146,42 @@ int set_err(cac_hip_server* s, int code, const std::string& m) {
     s->err = m;
   }
   return code;
+}
+
+// Read back the completed marks of a slot in order; wait for all of them
+// when `all`.  Each completion after the first of a replica adds the time
+// since the previous one to the device intervals.
+void harvest_marks(cac_hip_server* s, Slot& sl, bool all) {
+  std::vector<float> got;
+  while (sl.tail < sl.head) {
+    hipEvent_t e = sl.marks[sl.tail % MARKS];
+    if (all) {
+      if (hipEventSynchronize(e) != hipSuccess) break;
+    } else if (hipEventQuery(e) != hipSuccess) {
+      break;
+    }
+    if (sl.last >= 0) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, sl.marks[sl.last], e) == hipSuccess) {
+        got.push_back(ms);
+        sl.dev_sum += ms;
+        sl.dev_n++;
+      }
+    }
+    sl.last = (int)(sl.tail % MARKS);
+    sl.tail++;
+  }
+  if (!got.empty()) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->device_ms.insert(s->device_ms.end(), got.begin(), got.end());
+  }
+}
+
+// a step (or the load) of the slot's replica has been queued: mark its completion
+void mark_step(cac_hip_server* s, Slot& sl) {
+  if (sl.head - sl.tail >= MARKS - 1) harvest_marks(s, sl, true);
+  if (hipEventRecord(sl.marks[sl.head % MARKS], sl.h->c.stream) == hipSuccess) sl.head++;
+  harvest_marks(s, sl, false);
 }
 
 // load a request into a slot's context and set it up (neighbor build + first
@@ -184,6 +234,11 @@ int code_of(const HipError& e) {
 void finish(cac_hip_server* s, Active& a, int status, const std::string& msg) {
   Slot& sl = s->slots[a.slot];
   Ctx& c = sl.h->c;
+  harvest_marks(s, sl, true);
+  sl.last = -1;
+  const double dev_mean = sl.dev_n ? sl.dev_sum / (double)sl.dev_n : 0.0;
+  sl.dev_sum = 0.0;
+  sl.dev_n = 0;
   Outcome o;
   o.r.status = status;
   o.r.nsteps = a.req->r.nsteps;
@@ -192,6 +247,7 @@ void finish(cac_hip_server* s, Active& a, int status, const std::string& msg) {
   o.r.t_admit = a.t_admit;
   o.r.t_done = s->now();
   o.r.step_ms_mean = a.done > 0 ? 1000.0 * (o.r.t_done - a.t_first) / a.done : 0.0;
+  o.r.device_step_ms_mean = dev_mean;
   o.r.builds = c.ncalls;
   if (status == 0) {
     try {
@@ -224,6 +280,10 @@ void finish(cac_hip_server* s, Active& a, int status, const std::string& msg) {
   s->finished++;
   s->cv_done.notify_all();
   s->cv_work.notify_all();  // a slot is free for the loader
+  // an idle stepping thread waits for work or for the last replica in flight
+  // to finish once the loader is done: wake it, or it would sleep through
+  // the server's shutdown
+  s->cv_ready.notify_all();
 }
 
 // The loader thread: admits the head of the queue into a free slot when the
@@ -280,6 +340,7 @@ void load_loop(cac_hip_server* s) {
       finish(s, a, rc, msg);
       continue;
     }
+    mark_step(s, s->slots[slot]);  // the first step's interval starts when the load is done
     std::lock_guard<std::mutex> lk(s->mu);
     int best = 0;  // the stepping thread with the fewest replicas takes it
     for (int k = 1; k < (int)s->load_of.size(); k++)
@@ -315,6 +376,7 @@ void rebuild_loop(cac_hip_server* s) {
       finish(s, a, code_of(e), e.where);
       continue;
     }
+    mark_step(s, s->slots[a.slot]);
     const double t = s->now();
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -444,6 +506,7 @@ void serve(cac_hip_server* s, int me) {
         continue;
       }
       if (!a.issued) {
+        mark_step(s, s->slots[a.slot]);
         const double t = s->now();
         round_ms.push_back((float)(1000.0 * (t - a.t_last)));
         a.t_last = t;
@@ -451,6 +514,7 @@ void serve(cac_hip_server* s, int me) {
       }
     }
     auto stepped = [&](Active& a) {
+      mark_step(s, s->slots[a.slot]);
       const double t = s->now();
       round_ms.push_back((float)(1000.0 * (t - a.t_last)));
       a.t_last = t;
@@ -520,9 +584,16 @@ int cac_hip_server_create(cac_hip_server** out, int device, int slots, double hb
   s->t0 = Clock::now();
   for (int k = 0; k < slots; k++) {
     Slot sl;
-    const int rc = cac_hip_create(&sl.h, device, CAC_HIP_UNITS_METAL);
+    int rc = cac_hip_create(&sl.h, device, CAC_HIP_UNITS_METAL);
+    for (int m = 0; m < MARKS && !rc; m++)
+      if (hipEventCreate(&sl.marks[m]) != hipSuccess) rc = CAC_HIP_EHIP;
     if (rc) {
-      for (Slot& x : s->slots) cac_hip_destroy(x.h);
+      s->slots.push_back(sl);
+      for (Slot& x : s->slots) {
+        for (hipEvent_t e : x.marks)
+          if (e) (void)hipEventDestroy(e);
+        if (x.h) cac_hip_destroy(x.h);
+      }
       delete s;
       return rc;
     }
@@ -570,7 +641,11 @@ int cac_hip_server_destroy(cac_hip_server* s) {
   s->cv_rebuild.notify_all();
   for (std::thread& t : s->rebuilders)
     if (t.joinable()) t.join();
-  for (Slot& sl : s->slots) cac_hip_destroy(sl.h);
+  for (Slot& sl : s->slots) {
+    cac_hip_destroy(sl.h);  // synchronizes the slot's stream first
+    for (hipEvent_t e : sl.marks)
+      if (e) (void)hipEventDestroy(e);
+  }
   delete s;
   return CAC_HIP_OK;
 }
@@ -697,6 +772,22 @@ int cac_hip_server_segments(cac_hip_server* s, long* launches, long* replica_ste
   std::lock_guard<std::mutex> lk(s->mu);
   if (launches) *launches = s->seg_launches;
   if (replica_steps) *replica_steps = s->seg_steps;
+  return CAC_HIP_OK;
+}
+
+int cac_hip_server_device_intervals(cac_hip_server* s, double* p50_ms, double* p99_ms, long* n) {
+  if (!s) return CAC_HIP_EARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::vector<float> v = s->device_ms;
+  auto pct = [&](double q) -> double {
+    if (v.empty()) return 0.0;
+    const size_t k = std::min(v.size() - 1, (size_t)(q * (double)(v.size() - 1) + 0.5));
+    std::nth_element(v.begin(), v.begin() + (long)k, v.end());
+    return v[k];
+  };
+  if (p50_ms) *p50_ms = pct(0.50);
+  if (p99_ms) *p99_ms = pct(0.99);
+  if (n) *n = (long)s->device_ms.size();
   return CAC_HIP_OK;
 }
 

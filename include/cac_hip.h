@@ -214,6 +214,7 @@ typedef struct {
   double step_ms_mean;       /* (t_done - first step) / nsteps */
   double pe, ke, etotal;     /* thermo of the last step */
   long builds;               /* neighbor builds during the run */
+  double device_step_ms_mean; /* mean interval between the device's completions of consecutive steps */
 } cac_hip_result;
 
 /* slots: concurrent replicas (1..256); hbm_budget_bytes <= 0: 3/4 of free HBM */
@@ -238,6 +239,12 @@ int cac_hip_server_state(cac_hip_server *srv, long id, double *x, double *nodex)
  * any replica was active; peak slots in use; peak HBM the slots held */
 int cac_hip_server_stats(cac_hip_server *srv, double *p50_ms, double *p99_ms, long *nsteps, double *atom_steps_per_s,
                          int *peak_active, double *peak_hbm_bytes);
+/* step intervals as the device ran them: an event recorded on a replica's
+ * stream after its load and after each step marks the step's completion on
+ * the device; the time between consecutive completions of one replica (its
+ * kernels waiting behind other replicas' included) is one interval.  p50,
+ * p99 (ms) and the count over every replica step so far. */
+int cac_hip_server_device_intervals(cac_hip_server *srv, double *p50_ms, double *p99_ms, long *n);
 /* Segmented pair passes so far: groups launched (one density, one embedding
  * and one force launch each, over 2 to 16 replicas) and the replica steps they
  * covered.  With CAC_HIP_SERVE_SEGMENT=1 set when the server is created, a

@@ -140,3 +140,58 @@ def test_lj_replicas_in_lj_units_same_alone_and_in_the_mix():
         assert np.array_equal(x, x0)
         assert r["pe"] == th0[1] and r["ke"] == th0[2], (r, th0)
     srv.close()
+
+
+@pytest.mark.gpu
+def test_replicas_across_the_serving_range():
+    """Replicas drawn across the serving configuration's whole size range (2k
+    to 200k atom-equivalents, log-spaced), core/coarse ratio, temperature and
+    strain ranges, at its skin of 2.0 A, 100 mixed steps, four slots: none
+    fails; every step is timed on the device; the smallest, the median and
+    the largest are bitwise the same alone and in the mix."""
+    import cac_amd
+    rng = np.random.default_rng(99)
+    sizes = np.geomspace(2000.0, 200000.0, 7)
+    specs = [dict(arrival=0.0, size=float(sz), ratio=float(rng.uniform(0.05, 0.3)), T=float(rng.uniform(100, 600)),
+                  axis=int(rng.integers(0, 3)), strain=float(rng.uniform(-0.02, 0.02)),
+                  seed=int(rng.integers(0, 2 ** 31))) for sz in sizes]
+    dks = [decks.serving_deck(s) for s in specs]
+    ae = [d.atom_equivalents for d in dks]
+    assert min(ae) < 3000 and max(ae) > 150000, ae
+    nstep = 100
+    srv = cac_amd.Server(slots=4, eam_tables=TAB)
+    ids = [srv.submit(d, nstep, 0.001, skin=2.0, keep_state=True) for d in dks]
+    srv.drain()
+    st = srv.stats()
+    assert st["steps"] == len(dks) * nstep
+    assert st["device_steps"] == len(dks) * nstep  # one completion interval per step
+    assert 0.0 < st["device_p50_ms"] <= st["device_p99_ms"]
+    res = [srv.result(rid) for rid in ids]
+    assert all(r["status"] == 0 and r["nsteps"] == nstep for r in res), res
+    assert all(r["device_step_ms_mean"] > 0.0 for r in res)
+    for k in (0, len(dks) // 2, len(dks) - 1):
+        d = dks[k]
+        m = cac_amd.Model(d, "eam", skin=2.0, eam_tables=TAB, precision=cac_amd.MIXED)
+        m.setup()
+        th = m.run(nstep, 0.001)
+        x, nx = srv.state(ids[k], d)
+        assert np.array_equal(x, m.atoms()[0])
+        assert np.array_equal(nx, m.elements()["nodex"])
+        assert res[k]["pe"] == th[-1][1] and res[k]["ke"] == th[-1][2]
+        m.close()
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_server_closes_without_drain(monkeypatch):
+    """Closing a server whose replicas are still queued or running, with more
+    stepping threads than replicas (idle threads wait for work while the last
+    replica finishes on another), finishes their work and returns."""
+    import cac_amd
+    monkeypatch.setenv("CAC_HIP_SERVE_STEPPERS", "4")
+    d = decks.serving_deck(_specs(1, 5)[0])
+    for nrep in (1, 3):
+        srv = cac_amd.Server(slots=4, eam_tables=TAB)
+        for _ in range(nrep):
+            srv.submit(d, 60, 0.001, skin=SKIN)
+        srv.close()
