@@ -675,7 +675,10 @@ int cac_hip_pair_eam(cac_hip_ctx* h, int nrho, double drho, int nr, double dr, d
     // z2r's value coefficients and the derivative coefficients c0..c2 of rhor
     // and c2 of z2r (z2r's c0, c1 are 3 c3 / dr and 2 c4 / dr, formed in the
     // kernel), two float4 per knot
-    std::vector<float> t32r((size_t)(nr + 1) * 4), t32f((size_t)(nr + 1) * 8);
+    // one knot more than the reference's table: knot nr + 1 is all zeros, and
+    // the pair passes evaluate it for a list entry beyond the cutoff, so such
+    // an entry contributes an exact zero without a mask multiply
+    std::vector<float> t32r((size_t)(nr + 2) * 4, 0.0f), t32f((size_t)(nr + 2) * 8, 0.0f);
     for (int m = 0; m <= nr; m++) {
       for (int q = 0; q < 4; q++) {
         t32r[(size_t)m * 4 + q] = (float)rs[(size_t)m * 7 + 3 + q];
@@ -684,13 +687,13 @@ int cac_hip_pair_eam(cac_hip_ctx* h, int nrho, double drho, int nr, double dr, d
       for (int q = 0; q < 3; q++) t32f[(size_t)m * 8 + 4 + q] = (float)rs[(size_t)m * 7 + q];
       t32f[(size_t)m * 8 + 7] = (float)zs[(size_t)m * 7 + 2];
     }
-    std::vector<double> t64((size_t)(nr + 1) * 8);
+    std::vector<double> t64((size_t)(nr + 2) * 8, 0.0);
     for (int m = 0; m <= nr; m++)
       for (int q = 0; q < 4; q++) {
         t64[(size_t)m * 8 + q] = rs[(size_t)m * 7 + 3 + q];
         t64[(size_t)m * 8 + 4 + q] = zs[(size_t)m * 7 + 3 + q];
       }
-    if ((size_t)(nr + 1) * 64 > 160 * 1024) return fail(h, CAC_HIP_EARG, "nr too large for the LDS spline table");
+    if ((size_t)(nr + 2) * 64 > 160 * 1024) return fail(h, CAC_HIP_EARG, "nr too large for the LDS spline table");
     upload(c.tab32r, t32r, c.stream);
     upload(c.tab32f, t32f, c.stream);
     upload(c.tab64, t64, c.stream);

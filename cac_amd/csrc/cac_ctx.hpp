@@ -36,6 +36,9 @@
 namespace cac {
 
 constexpr int NPE = 8;
+// inner-list rows are padded to a multiple of ROW_PAD entries with entries
+// naming a dummy record beyond every cutoff; the pair passes read whole batches
+constexpr int ROW_PAD = 16;
 constexpr int IMG_SHIFT = 27;
 constexpr unsigned SITE_MASK = (1u << IMG_SHIFT) - 1u;  // list entry = site | image << 27, read as unsigned
 
@@ -227,6 +230,7 @@ struct Ctx {
   DBuf<int> row_idx, row_tmp;
   DBuf<unsigned char> sort_tmp;
   DBuf<int> nnum, nlist;
+  DBuf<int> nearrow;  // per row: a candidate lies within the near-cutoff band
   int cap = 0;
   DBuf<int> innum, ilist;
   int icap = 0;

@@ -356,7 +356,7 @@ __device__ __forceinline__ bool gauss_pair_in_row(const RefOrder& ro, const doub
   return true;
 }
 
-__device__ __noinline__ double reference_rsq(const double* __restrict__ x, int P, int A, const UcellGrid& ug,
+__device__ __forceinline__ double reference_rsq(const double* __restrict__ x, int P, int A, const UcellGrid& ug,
                                              const RefOrder& ro, const double* L, int prow, int wrow, int dsite,
                                              int wsite, unsigned tcode) {
 #pragma clang fp contract(off)
@@ -409,17 +409,28 @@ __device__ __noinline__ double reference_rsq(const double* __restrict__ x, int P
 // shared status words.
 constexpr int BL = 8;
 
+//
+// The scan runs twice.  The first pass (REF false) keeps a candidate by the
+// record difference alone and marks, in nearrow, each row holding a candidate
+// within NEAR_CUT_REL of the cutoff; the second (REF true) visits only the
+// marked rows and scans them again from the start, deciding the near
+// candidates by reference_rsq.  The rare recompute then costs the common scan
+// no registers.
+template <bool REF>
 __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__ site, const int* __restrict__ row_site,
                                                      int R, Grid g, const int* __restrict__ cstart, double cutsq,
                                                      int* __restrict__ nnum, unsigned* __restrict__ nlist, int cap,
-                                                     ErrorFlags* flags, const double* __restrict__ x, int P, int A,
-                                                     UcellGrid ug, RefOrder ro, const int* __restrict__ sdesc,
+                                                     ErrorFlags* flags, int* __restrict__ nearrow,
+                                                     const double* __restrict__ x, int P, int A, UcellGrid ug,
+                                                     RefOrder ro, const int* __restrict__ sdesc,
                                                      const int* __restrict__ simg) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x % BL;
   const int gbase = (threadIdx.x & 63) & ~(BL - 1);
   const int r = (int)(((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / BL);
   if (r >= R) return;
+  if (REF && !nearrow[r]) return;
+  bool nearhit = false;
   const int k = row_site[r];
   const double4 pk = site[k];
   const double px = pk.x, py = pk.y, pz = pk.z;
@@ -468,8 +479,10 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
             const double dely = py - (sj.y + shy);
             const double delz = pz - (sj.z + shz);
             double rsq = delx * delx + dely * dely + delz * delz;
-            if (fabs(rsq - cutsq) <= near)
-              rsq = reference_rsq(x, P, A, ug, ro, g.L, sdesc[k], simg[k], sdesc[j], simg[j], tcode);
+            if (fabs(rsq - cutsq) <= near) {
+              if (REF) rsq = reference_rsq(x, P, A, ug, ro, g.L, sdesc[k], simg[k], sdesc[j], simg[j], tcode);
+              else nearhit = true;
+            }
             keep = rsq <= cutsq && !(j == k && tcode == 13u);
           }
           const unsigned long long m = __ballot(keep);
@@ -480,6 +493,10 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
         }
       }
     }
+  }
+  if (!REF) {
+    const unsigned long long h = __ballot(nearhit);
+    if (lane == 0) nearrow[r] = ((h >> gbase) & ((1ull << BL) - 1ull)) != 0ull;
   }
   if (lane == 0) {
     nnum[r] = n < cap ? n : cap;
@@ -625,6 +642,7 @@ void build_neighbor(Ctx& c) {
     c.cap = std::min(c.one, ((int)(1.5 * est) + 32 + 15) / 16 * 16);
   }
   c.nnum.reserve(P);
+  c.nearrow.reserve(P);
   RefOrder ro;
   ro.ex = c.E ? c.ex.get() : nullptr;
   ro.E = c.E;
@@ -632,11 +650,14 @@ void build_neighbor(Ctx& c) {
   for (int attempt = 0; attempt < 3; attempt++) {
     c.nlist.reserve((size_t)c.cap * P);
     CAC_HIP(hipMemsetAsync(&c.d_flags.get()->overflow, 0, sizeof(int) * 2, c.stream));
-    k_build_lists<<<grid_for((long)BL * P, 256), 256, 0, c.stream>>>(c.site.get(), c.row_site.get(), P, g, c.cell_start.get(),
-                                                          c.cutneighsq, c.nnum.get(),
-                                                          reinterpret_cast<unsigned*>(c.nlist.get()), c.cap,
-                                                          c.d_flags.get(), c.x.get(), P, c.A, ucell_grid(c), ro,
-                                                          c.sdesc.get(), c.simg.get());
+    const UcellGrid ug = ucell_grid(c);
+    unsigned* nl = reinterpret_cast<unsigned*>(c.nlist.get());
+    k_build_lists<false><<<grid_for((long)BL * P, 256), 256, 0, c.stream>>>(
+        c.site.get(), c.row_site.get(), P, g, c.cell_start.get(), c.cutneighsq, c.nnum.get(), nl, c.cap,
+        c.d_flags.get(), c.nearrow.get(), c.x.get(), P, c.A, ug, ro, c.sdesc.get(), c.simg.get());
+    k_build_lists<true><<<grid_for((long)BL * P, 256), 256, 0, c.stream>>>(
+        c.site.get(), c.row_site.get(), P, g, c.cell_start.get(), c.cutneighsq, c.nnum.get(), nl, c.cap,
+        c.d_flags.get(), c.nearrow.get(), c.x.get(), P, c.A, ug, ro, c.sdesc.get(), c.simg.get());
     read_flags(c);
     if (!c.h_flags->overflow) break;
     int need = c.h_flags->maxcount;
@@ -751,7 +772,7 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
                                                const int* __restrict__ row_site, const int* __restrict__ row_point,
                                                const int* __restrict__ nnum, const unsigned* __restrict__ nlist,
                                                int R, int cap, Grid g, double cutsq_in, int* __restrict__ innum,
-                                               unsigned* __restrict__ ilist, const double* __restrict__ x,
+                                               unsigned* __restrict__ ilist, int icap, const double* __restrict__ x,
                                                double* __restrict__ xhold_in, int* __restrict__ live) {
   constexpr int PU = MIXED ? 9 : 6;
   // LDS tables of the mixed test: the cell shift of each of the 27 image codes,
@@ -791,7 +812,7 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
     const int cp = MIXED ? __float_as_int(pk32.w) : 0;
     const int icx = cell_ix(cp), icy = cell_iy(cp), icz = cell_iz(cp);
     const unsigned* row = nlist + (size_t)r * cap;
-    unsigned* out = ilist + (size_t)r * cap;
+    unsigned* out = ilist + (size_t)r * icap;
     int count = 0;
     for (int base = 0; base < n; base += 16 * PU) {
       unsigned ent[PU];
@@ -856,11 +877,24 @@ __global__ void k_live_count(const int* __restrict__ flag, const int* __restrict
   if (blockIdx.x == 0 && threadIdx.x == 0) *n = S ? idx[S - 1] + flag[S - 1] : 0;
 }
 
+// Each row is then padded to a multiple of ROW_PAD entries (the pair passes
+// read whole batches) with an entry naming the dummy record at index nlive,
+// far from every point, which the first thread writes: the pair passes find
+// it beyond every cutoff.
+constexpr double DUMMY_COORD = 1e5;  // dummy record coordinates (A or sigma)
+
 __global__ __launch_bounds__(256) void k_live_remap(const int* __restrict__ idx, const int* __restrict__ row_site,
                                                     int* __restrict__ row_live, const int* __restrict__ innum,
-                                                    unsigned* __restrict__ ilist, int R, int cap, unsigned mask) {
+                                                    unsigned* __restrict__ ilist, int R, int cap, unsigned mask,
+                                                    unsigned pad_code, const int* __restrict__ nlive,
+                                                    float4* __restrict__ site32, double4* __restrict__ site_live) {
   constexpr int RU = 6;
   const int lane = threadIdx.x & 15;
+  const unsigned dummy = (unsigned)*nlive | pad_code;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (site32) site32[*nlive] = make_float4((float)DUMMY_COORD, (float)DUMMY_COORD, (float)DUMMY_COORD, 0.0f);
+    if (site_live) site_live[*nlive] = make_double4(DUMMY_COORD, DUMMY_COORD, DUMMY_COORD, 0.0);
+  }
   const long rstride = ((long)gridDim.x * blockDim.x) >> 4;
   const long rfirst = ((long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 4;
   for (long rl = rfirst; rl < R; rl += rstride) {
@@ -884,6 +918,8 @@ __global__ __launch_bounds__(256) void k_live_remap(const int* __restrict__ idx,
         if (i < n) row[i] = (unsigned)d[u] | (e[u] & ~mask);
       }
     }
+    const int np = (n + ROW_PAD - 1) / ROW_PAD * ROW_PAD;  // <= icap, a multiple of 16 >= n
+    if (n + lane < np) row[n + lane] = dummy;
   }
 }
 
@@ -894,7 +930,7 @@ void build_inner(Ctx& c) {
   const double rin = c.cutforce + c.inner_skin;
   c.inner_cutsq = rin * rin;
   Grid g = grid_of(c);
-  c.icap = c.cap;
+  c.icap = (c.cap + 15) / 16 * 16;  // a row padded to a multiple of ROW_PAD entries fits
   // the pair passes address the inner list with 32-bit byte offsets
   if ((size_t)c.P * (size_t)c.icap * 4 >= (1ull << 32))
     throw HipError{hipErrorInvalidValue, "inner neighbor list larger than 4 GiB (too many points for one GPU)"};
@@ -911,16 +947,20 @@ void build_inner(Ctx& c) {
   if (mixed)
     k_prune<true><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.row_site.get(), c.row_point.get(),
                                                  c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
-                                                 c.x.get(), c.xhold_in.get(), c.live_flag.get());
+                                                 c.icap, c.x.get(), c.xhold_in.get(), c.live_flag.get());
   else
     k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.row_site.get(), c.row_point.get(),
                                                   c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
-                                                  c.x.get(), c.xhold_in.get(), c.live_flag.get());
+                                                  c.icap, c.x.get(), c.xhold_in.get(), c.live_flag.get());
   scan_exclusive(c, c.live_flag.get(), c.live_idx.get(), c.S, nullptr);
   k_live_count<<<1, 64, 0, c.stream>>>(c.live_flag.get(), c.live_idx.get(), c.S, c.d_nlive.get());
   compact_live(c);
+  // the dummy entry: no cell offset (mixed) / no image shift (FP64)
+  const unsigned pad_code = mixed ? (62u << MIX_SHIFT) : (13u << IMG_SHIFT);
   k_live_remap<<<blocks, 256, 0, c.stream>>>(c.live_idx.get(), c.row_site.get(), c.row_live.get(), c.innum.get(), il,
-                                             c.P, c.icap, mixed ? (1u << MIX_SHIFT) - 1u : SITE_MASK);
+                                             c.P, c.icap, mixed ? (1u << MIX_SHIFT) - 1u : SITE_MASK, pad_code,
+                                             c.d_nlive.get(), mixed ? c.site32.get() : nullptr,
+                                             mixed ? nullptr : c.site_live.get());
   c.inner_mode = c.precision;
   c.inner_builds++;
 }

@@ -47,17 +47,26 @@ struct PairArgs {
   long nchunks;
 };
 
+// A mixed-path list entry is site | c << 25: the site image lies d = (dx, dy,
+// dz) cells from the row's cell, each component in -2..2, and
+// c = (dx + 2) + 5 (dy + 2) + 25 (dz + 2).
 constexpr int MIX_SHIFT = 25;
 constexpr unsigned MIX_MASK = (1u << MIX_SHIFT) - 1u;
 
-// mixed-path pair vector: site image j lies d = (dx, dy, dz) cells from row i's
-// cell, so x_i - x_j = (u_i - u_j) - d * cell edge with u relative to each
-// site's cell corner
+// mixed-path pair vector: x_i - x_j = (u_i - u_j) - d * cell edge with u
+// relative to each site's cell corner.  d * edge is exact in FP32 (|d| <= 2),
+// so each component rounds once.  d is decoded from c in FP32 with no memory
+// access: dz = floor((c + 1/2) / 25) - 2, then t = c - 25 dz = (dx + 2) +
+// 5 (dy + 2) + 50, dy = floor((t - 50 + 1/2) / 5) - 2 and dx = t - 5 dy - 62.
+// Every quotient lies at least 0.02 from an integer, far beyond the rounding
+// of the FP32 multiply-adds, so the floors are exact.
 __device__ __forceinline__ void mixed_delta(unsigned ent, const float4& pi, const float4& sj, const float* cs,
                                             float& dx, float& dy, float& dz) {
-  const unsigned dc = ent >> MIX_SHIFT;
-  const float ox = (float)((int)(dc % 5u) - 2), oy = (float)((int)((dc / 5u) % 5u) - 2),
-              oz = (float)((int)(dc / 25u) - 2);
+  const float c = (float)(ent >> MIX_SHIFT);
+  const float oz = __builtin_floorf(__builtin_fmaf(c, 0.04f, -1.98f));
+  const float t = __builtin_fmaf(oz, -25.0f, c);
+  const float oy = __builtin_floorf(__builtin_fmaf(t, 0.2f, -11.9f));
+  const float ox = __builtin_fmaf(oy, -5.0f, t) - 62.0f;
   dx = (pi.x - sj.x) - ox * cs[0];
   dy = (pi.y - sj.y) - oy * cs[1];
   dz = (pi.z - sj.z) - oz * cs[2];
@@ -109,19 +118,6 @@ __device__ __forceinline__ double row_sum(double v) {
   return v;
 }
 
-// cell-offset table of the mixed path in LDS: code (dx+2) + 5(dy+2) + 25(dz+2)
-// -> (dx, dy, dz) * cell edge, so the pair vector is (u_i - u_j) - offset
-constexpr int NOFF = 125;
-// the mixed passes keep this table at the start of LDS (a lookup is then a
-// shift and a load) and their spline table NOFF_PAD float4 after it
-constexpr int NOFF_PAD = 128;
-
-__device__ __forceinline__ void stage_offsets(float4* offs, const float* cs) {
-  for (int t = threadIdx.x; t < NOFF; t += blockDim.x) {
-    const int ox = t % 5 - 2, oy = (t / 5) % 5 - 2, oz = t / 25 - 2;
-    offs[t] = make_float4((float)ox * cs[0], (float)oy * cs[1], (float)oz * cs[2], 0.0f);
-  }
-}
 
 // ------------------------------------------------------------------ site refresh
 // forward_comm equivalent: every live site's position from its source point,
@@ -259,15 +255,16 @@ static Frame frame_of(const Ctx& c) {
   return fr;
 }
 
+// dense records: every live site, plus the dummy record the padded list rows name
 static void reserve_live(Ctx& c) {
   c.lsdesc.reserve(c.S);
   c.lsimg.reserve(c.S);
   c.lscell.reserve(c.S);
   if (c.precision == 1) {
-    c.site32.reserve(c.S);
+    c.site32.reserve((size_t)c.S + 1);
     c.site32f.reserve(c.S);
   } else {
-    c.site_live.reserve(c.S);
+    c.site_live.reserve((size_t)c.S + 1);
   }
 }
 
@@ -318,15 +315,19 @@ __device__ __forceinline__ int knot_r(T r, T rdr, int nr, T& p) {
   return m;
 }
 
-// FP32 knot of the mixed path; r >= 0 keeps m >= 1, r < cut keeps m <= nr - 1
-// up to rounding, which the clamp covers
-__device__ __forceinline__ int knot_r32(float r, float rdr, int nr, float& p) {
-  p = r * rdr + 1.0f;
-  int m = (int)p;
-  m = m < nr - 1 ? m : nr - 1;
-  p -= (float)m;
-  p = p < 1.0f ? p : 1.0f;
-  return m;
+// FP32 knot of the mixed path: m = int(r rdr + 1), p its fraction, as the
+// reference forms them for r < (nr - 1) dr.  Past that (the last knot
+// interval, where the tabulated functions of a potential that reaches zero
+// at its cutoff are near zero) the reference clamps to knot nr - 1 at p = 1;
+// here knot nr at its fraction continues the spline linearly, and m is kept
+// at most nr + 1 (the all-zero knot), so a table read stays inside the table.
+// An entry beyond the cutoff takes the zero knot: the caller's `inside`.
+__device__ __forceinline__ int knot_r32(float r, float rdr, int nr, bool inside, float& p) {
+  const float x = r * rdr + 1.0f;
+  p = __builtin_amdgcn_fractf(x);
+  int m = (int)x;
+  m = m < nr + 1 ? m : nr + 1;
+  return inside ? m : nr + 1;
 }
 
 __device__ __forceinline__ int knot_rho(double rho, double rdrho, int nrho, double& p) {
@@ -339,13 +340,23 @@ __device__ __forceinline__ int knot_rho(double rho, double rdrho, int nrho, doub
   return m;
 }
 
-// spline table in LDS: knot m holds (rhor c3..c6) at 2m and (z2r c3..c6) at 2m+1,
-// float4 for the mixed path, double4 for FP64; derivatives are formed from
-// these value coefficients, ((3 c3 p + 2 c4) p + c5) / dr, which is what the
-// reference's coefficients c0..c2 hold
+// spline tables: knot m holds two vectors at 2m and 2m + 1 in device memory
+// (FP64: rhor c3..c6 and z2r c3..c6; mixed: z2r c3..c6, then rhor's
+// derivative coefficients c0..c2 and z2r's c2); derivatives are formed from the
+// value coefficients, ((3 c3 p + 2 c4) p + c5) / dr, which is what the
+// reference's coefficients c0..c2 hold.  The density pass stages the rhor
+// half alone; the force pass stages both halves as two tables of n vectors
+// each, so the lanes' reads of unrelated knots spread over all LDS banks (a
+// 16-byte stride for the mixed path) rather than every other bank group.
 template <typename V>
 __device__ __forceinline__ void stage_table(V* lds, const V* __restrict__ g, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+  __syncthreads();
+}
+
+template <typename V>
+__device__ __forceinline__ void stage_halves(V* lds, const V* __restrict__ g, int n) {
+  for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) lds[(i & 1) * n + (i >> 1)] = g[i];
   __syncthreads();
 }
 
@@ -356,58 +367,69 @@ __device__ __forceinline__ void stage_table_strided(V* lds, const V* __restrict_
 }
 
 // ------------------------------------------------------------------ row walker
-// Lane l of a row's TPR lanes visits entries l, l + TPR, ...; U entries at a
-// time: U list loads, then U site gathers, then the U pair terms, so U
-// independent memory chains are in flight per lane instead of one.  The pair
-// terms take a validity flag instead of a branch (an entry past the row's end
-// or outside the cutoff contributes an exact zero), so the U terms form one
-// straight-line block the compiler can interleave.
+// The list build pads every inner row to a multiple of 16 entries with an
+// entry naming a dummy site far from every point, so every word a lane loads
+// up to the row's length rounded up to 2 TPR lies inside its row: no index
+// clamp and no row-end mask.  An entry beyond the cutoff (a dummy among them)
+// evaluates the all-zero spline knot nr + 1 appended to the tables, so it
+// contributes an exact zero.
+//
+// Lane l of a row's TPR lanes takes entries l, l + TPR, ...: full batches of
+// U entries per lane (U list loads, then U site gathers, then U pair terms,
+// so U independent memory chains are in flight), then the padded tail in
+// batches of 2 entries per lane.  `flush` runs after every GROUP entries of a
+// lane and after its last one (the passes move their FP32 group sums into
+// FP64 sums there); groups follow the lane's entry count, not the batches, so
+// the sums do not depend on U.
+constexpr int UNROLL = 4;  // entries per lane in flight in a full batch
+constexpr int GROUP = 4;   // entries of a lane summed in FP32 before the FP64 sum
 
-constexpr int UNROLL = 4;
-
-// U entries per lane per batch: lane l of the row takes entries l, l + TPR,
-// ...  List words are read at an index clamped into the row (n >= 1 here):
-// past the row's end a lane re-reads its last entry, a real site, which the
-// body masks out.  The load is always in bounds and needs no branch or select.
-// The list is addressed by 32-bit byte offsets from its base (a scalar
-// pointer), so a word load is one shift and a load: the host keeps every list
-// below 4 GiB.
-__device__ __forceinline__ unsigned list_word(const unsigned* __restrict__ base, unsigned byte_off) {
-  return *reinterpret_cast<const unsigned*>(reinterpret_cast<const char*>(base) + byte_off);
-}
-
-template <int TPR, int U>
-__device__ __forceinline__ void row_words(const unsigned* __restrict__ base, unsigned row0, int n, int j0,
-                                          unsigned* e) {
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int jj = j0 + u * TPR;
-    e[u] = list_word(base, (row0 + (unsigned)(jj < n ? jj : n - 1)) << 2);
-  }
-}
-
-// row r of the list starts at word row0 = r * cap
-template <int TPR, int U, typename Site, typename Fetch, typename Body>
+template <int TPR, int U, typename Site, typename Fetch, typename Body, typename Flush>
 __device__ __forceinline__ void walk_row(const unsigned* __restrict__ base, unsigned row0, int n, int lane,
-                                         Fetch fetch, Body body) {
-  for (int j0 = lane; j0 < n; j0 += U * TPR) {
-    unsigned e[U];
-    row_words<TPR, U>(base, row0, n, j0, e);
+                                         Fetch fetch, Body body, Flush flush) {
+  constexpr int UT = 2;  // entries per lane in a tail batch
+  constexpr int B = U * TPR;
+  static_assert(ROW_PAD % (UT * TPR) == 0, "tail batches must stay inside the padded row");
+  static_assert(U % GROUP == 0, "a full batch holds whole groups");
+  const unsigned* __restrict__ rp = base + row0 + lane;
+  const int np = (n + UT * TPR - 1) / (UT * TPR) * (UT * TPR);
+  const int nfull = np / B;
+  for (int b = 0; b < nfull; b++) {
+    unsigned w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) w[u] = rp[b * B + u * TPR];
     Site sj[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) sj[u] = fetch(e[u]);
+    for (int u = 0; u < U; u++) sj[u] = fetch(w[u]);
 #pragma unroll
-    for (int u = 0; u < U; u++) body(e[u], sj[u], j0 + u * TPR < n);
+    for (int u = 0; u < U; u++) {
+      body(w[u], sj[u]);
+      if ((u + 1) % GROUP == 0) flush();
+    }
   }
+  static_assert(GROUP % UT == 0, "tail batches fill whole groups");
+  int cnt = 0;  // the lane's entries since its last full batch
+  for (int j = nfull * B; j < np; j += UT * TPR) {
+    unsigned e[UT];
+    Site sj[UT];
+#pragma unroll
+    for (int u = 0; u < UT; u++) e[u] = rp[j + u * TPR];
+#pragma unroll
+    for (int u = 0; u < UT; u++) sj[u] = fetch(e[u]);
+#pragma unroll
+    for (int u = 0; u < UT; u++) body(e[u], sj[u]);
+    cnt += UT;
+    if (cnt % GROUP == 0) flush();
+  }
+  if (cnt % GROUP) flush();
 }
 
 // ------------------------------------------------------------------ EAM density (pair_eam.cpp:721-813)
 
 // Density of the rows of chunk ch (NT / TPR rows, TPR lanes each) of one
-// model.  offs holds the model's cell-offset vectors (mixed path); the spline
-// table is staged in LDS by the caller.
+// model; the spline table is staged in LDS by the caller.
 template <int TPR, bool MIXED, int NT, int U>
-__device__ __forceinline__ void density_chunk(const PairArgs& a, long ch, const float4* offs, const float4* tab32,
+__device__ __forceinline__ void density_chunk(const PairArgs& a, long ch, const float4* tab32,
                                               const double4* lds64, double rdr, int nr,
                                               double* __restrict__ rho_out) {
   const int lane = threadIdx.x % TPR;
@@ -422,39 +444,41 @@ __device__ __forceinline__ void density_chunk(const PairArgs& a, long ch, const 
     const unsigned row0 = (unsigned)r * (unsigned)a.cap;
     if (MIXED) {
       const float4 pi = a.site32[k];
+      float acc32 = 0.0f;
       walk_row<TPR, U, float4>(
           a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
-          [&](unsigned ent, const float4& sj, bool valid) {
-            const float4 o = offs[ent >> MIX_SHIFT];
-            const float dx = (pi.x - sj.x) - o.x;
-            const float dy = (pi.y - sj.y) - o.y;
-            const float dz = (pi.z - sj.z) - o.z;
+          [&](unsigned ent, const float4& sj) {
+            float dx, dy, dz;
+            mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
             const float rsq = dx * dx + dy * dy + dz * dz;
-            // entries past the row's end or beyond rc count 0 by a mask
-            // multiply: a select would become a branch per entry
-            const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
             float p;
-            const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, nr, p);
+            // beyond rc (or the dummy): the zero knot
+            const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, nr, rsq < cut32, p);
             const float4 c = tab32[m];
-            acc += (double)(okf * (((c.x * p + c.y) * p + c.z) * p + c.w));
+            acc32 += ((c.x * p + c.y) * p + c.z) * p + c.w;
+          },
+          [&]() {
+            acc += (double)acc32;  // a lane's group of at most four terms, summed in FP32
+            acc32 = 0.0f;
           });
     } else {
       const double4 pi = a.site[k];
       walk_row<TPR, U, double4>(
           a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
-          [&](unsigned ent, const double4& sj, bool valid) {
+          [&](unsigned ent, const double4& sj) {
             double shx, shy, shz;
             img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
             const double delx = pi.x - (sj.x + shx);
             const double dely = pi.y - (sj.y + shy);
             const double delz = pi.z - (sj.z + shz);
             const double rsq = delx * delx + dely * dely + delz * delz;
-            const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
             double p;
-            const int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
+            int m = knot_r<double>(sqrt(rsq), rdr, nr, p);
+            m = rsq < a.cutsq ? m : nr + 1;
             const double4 c = lds64[m];
-            acc += okf * (((c.x * p + c.y) * p + c.z) * p + c.w);
-          });
+            acc += ((c.x * p + c.y) * p + c.z) * p + c.w;
+          },
+          []() {});
     }
   }
   acc = row_sum<TPR>(acc);
@@ -466,19 +490,17 @@ template <int TPR, bool MIXED, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_density(PairArgs a, const void* __restrict__ tab, double rdr, int nr,
                                                     double* __restrict__ rho_out, Gate g) {
   if (g.skip()) return;
-  __shared__ float4 offs[NOFF_PAD];  // static: its address is a constant of the code
   extern __shared__ float4 lds_raw[];
   float4* tab32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
   if (MIXED) {
-    stage_offsets(offs, a.cs);
-    stage_table(tab32, static_cast<const float4*>(tab), nr + 1);
+    stage_table(tab32, static_cast<const float4*>(tab), nr + 2);
   } else {
-    stage_table_strided(lds64, static_cast<const double4*>(tab), nr + 1);
+    stage_table_strided(lds64, static_cast<const double4*>(tab), nr + 2);
   }
   const Sched s = sched(a.nchunks);
   for (long ch = s.first; ch < s.end; ch += s.stride)
-    density_chunk<TPR, MIXED, NT, U>(a, ch, offs, tab32, lds64, rdr, nr, rho_out);
+    density_chunk<TPR, MIXED, NT, U>(a, ch, tab32, lds64, rdr, nr, rho_out);
 }
 
 // ------------------------------------------------------------------ embedding derivative per site
@@ -560,8 +582,7 @@ struct EamForceArgs {
 // Force (and, with EV, energy and virial) of the rows of chunk ch of one
 // model; as density_chunk.
 template <int TPR, bool MIXED, bool EV, int NT, int U>
-__device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArgs& q, long ch, const float4* offs,
-                                            const float4* tab32, const double4* lds64, double* __restrict__ f,
+__device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArgs& q, long ch, const float4* tab32, const double4* lds64, double* __restrict__ f,
                                             double* __restrict__ gf, int P, int E8, double* __restrict__ erow) {
   const int lane = threadIdx.x % TPR;
   const int rsub = threadIdx.x / TPR;
@@ -577,23 +598,20 @@ __device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArg
     if (MIXED) {
       const float4 pi = a.site32[k];
       const float cut32 = (float)a.cutsq;
+      float fx32 = 0.0f, fy32 = 0.0f, fz32 = 0.0f;
       walk_row<TPR, U, float4>(
           a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
-          [&](unsigned ent, const float4& sj, bool valid) {
-            const float4 o = offs[ent >> MIX_SHIFT];
-            const float dx = (pi.x - sj.x) - o.x;
-            const float dy = (pi.y - sj.y) - o.y;
-            const float dz = (pi.z - sj.z) - o.z;
-            // rsq is floored so that a masked entry (the row's own site past
-            // the row's end) stays finite and contributes 0 * finite
-            const float rsq = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
-            const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
+          [&](unsigned ent, const float4& sj) {
+            float dx, dy, dz;
+            mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
+            const float rsq = dx * dx + dy * dy + dz * dz;
             const float recip = __builtin_amdgcn_rsqf(rsq);
             const float rr = rsq * recip;
             float p;
-            const int m = knot_r32(rr, rdr32, q.nr, p);
-            const float4 cz = tab32[2 * m];      // z2r value c3..c6
-            const float4 cd = tab32[2 * m + 1];  // rhor derivative c0..c2, z2r c2
+            // beyond rc (or the dummy): the zero knot
+            const int m = knot_r32(rr, rdr32, q.nr, rsq < cut32, p);
+            const float4 cz = tab32[m];             // z2r value c3..c6
+            const float4 cd = tab32[q.nr + 2 + m];  // rhor derivative c0..c2, z2r c2
             const float rhop = (cd.x * p + cd.y) * p + cd.z;
             // z2 = ((c3 p + c4) p + c5) p + c6 by Horner, and dz2/dp from
             // the same partial sums; z2' = dz2/dp * rdr
@@ -603,13 +621,12 @@ __device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArg
             const float dz2 = (cz.x * p + h1) * p + h2;
             const float phi = z2 * recip;
             const float phip = (dz2 * rdr32 - phi) * recip;
-            const float fpair = okf * (-((pi.w + sj.w) * rhop + phip) * recip);
-            const double fp = (double)fpair;
-            fx = __builtin_fma((double)dx, fp, fx);
-            fy = __builtin_fma((double)dy, fp, fy);
-            fz = __builtin_fma((double)dz, fp, fz);
+            const float fpair = -((pi.w + sj.w) * rhop + phip) * recip;
+            fx32 = __builtin_fmaf(dx, fpair, fx32);
+            fy32 = __builtin_fmaf(dy, fpair, fy32);
+            fz32 = __builtin_fmaf(dz, fpair, fz32);
             if (EV) {
-              ephi += (double)(okf * phi);
+              ephi += (double)phi;
               v0 += (double)(dx * dx * fpair);
               v1 += (double)(dy * dy * fpair);
               v2 += (double)(dz * dz * fpair);
@@ -617,24 +634,32 @@ __device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArg
               v4 += (double)(dx * dz * fpair);
               v5 += (double)(dy * dz * fpair);
             }
+          },
+          [&]() {
+            // a lane's group of at most four pair forces is summed in FP32,
+            // the point's force in FP64
+            fx += (double)fx32;
+            fy += (double)fy32;
+            fz += (double)fz32;
+            fx32 = fy32 = fz32 = 0.0f;
           });
     } else {
       const double4 pi = a.site[k];
       walk_row<TPR, U, double4>(
           a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
-          [&](unsigned ent, const double4& sj, bool valid) {
+          [&](unsigned ent, const double4& sj) {
             double shx, shy, shz;
             img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
             const double delx = pi.x - (sj.x + shx);
             const double dely = pi.y - (sj.y + shy);
             const double delz = pi.z - (sj.z + shz);
-            const double rsq = fmax(delx * delx + dely * dely + delz * delz, 1e-200);
-            const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
+            const double rsq = delx * delx + dely * dely + delz * delz;
             const double rr = sqrt(rsq);
             double p;
-            const int m = knot_r<double>(rr, q.rdr, q.nr, p);
-            const double4 cr = lds64[2 * m];
-            const double4 cz = lds64[2 * m + 1];
+            int m = knot_r<double>(rr, q.rdr, q.nr, p);
+            m = rsq < a.cutsq ? m : q.nr + 1;  // beyond rc (or the dummy): the zero knot
+            const double4 cr = lds64[m];
+            const double4 cz = lds64[q.nr + 2 + m];
             const double rhop = ((3.0 * cr.x * p + 2.0 * cr.y) * p + cr.z) * q.rdr;
             const double z2p = ((3.0 * cz.x * p + 2.0 * cz.y) * p + cz.z) * q.rdr;
             const double z2 = ((cz.x * p + cz.y) * p + cz.z) * p + cz.w;
@@ -642,12 +667,12 @@ __device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArg
             const double phi = z2 * recip;
             const double phip = z2p * recip - phi * recip;
             const double psip = pi.w * rhop + sj.w * rhop + phip;
-            const double fpair = okf * (-1.0 * psip * recip);
+            const double fpair = -1.0 * psip * recip;
             fx += delx * fpair;
             fy += dely * fpair;
             fz += delz * fpair;
             if (EV) {
-              ephi += okf * phi;
+              ephi += phi;
               v0 += delx * delx * fpair;
               v1 += dely * dely * fpair;
               v2 += delz * delz * fpair;
@@ -655,7 +680,8 @@ __device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArg
               v4 += delx * delz * fpair;
               v5 += dely * delz * fpair;
             }
-          });
+          },
+          []() {});
     }
   }
   fx = row_sum<TPR>(fx);
@@ -720,19 +746,17 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
                                                   double* __restrict__ gf, int P, int E8,
                                                   double* __restrict__ erow, Gate g) {
   if (g.skip()) return;
-  __shared__ float4 offs[NOFF_PAD];  // static: its address is a constant of the code
   extern __shared__ float4 lds_raw[];
   float4* tab32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
   if (MIXED) {
-    stage_offsets(offs, a.cs);
-    stage_table(tab32, static_cast<const float4*>(q.tab), 2 * (q.nr + 1));
+    stage_halves(tab32, static_cast<const float4*>(q.tab), q.nr + 2);
   } else {
-    stage_table(lds64, static_cast<const double4*>(q.tab), 2 * (q.nr + 1));
+    stage_halves(lds64, static_cast<const double4*>(q.tab), q.nr + 2);
   }
   const Sched s = sched(a.nchunks);
   for (long ch = s.first; ch < s.end; ch += s.stride)
-    force_chunk<TPR, MIXED, EV, NT, U>(a, q, ch, offs, tab32, lds64, f, gf, P, E8, erow);
+    force_chunk<TPR, MIXED, EV, NT, U>(a, q, ch, tab32, lds64, f, gf, P, E8, erow);
 }
 
 // ------------------------------------------------------------------ several models in one launch
@@ -740,8 +764,7 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
 // models that share the potential and the launch shape, one launch each (the
 // serving loop's replicas).  The models' chunks of rows (blocks of live sites
 // for the embedding) are laid end to end; a workgroup finds the model of each
-// chunk from the running counts and, when the model changes, stages that
-// model's cell-offset vectors.  A row is walked by the same lanes in the same
+// chunk from the running counts.  A row is walked by the same lanes in the same
 // order as in the model's own launch, so its sums are bitwise those of a lone
 // run.
 
@@ -777,41 +800,25 @@ __device__ __forceinline__ int seg_of(const long* first, int n, long ch) {
 template <int TPR, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_density_seg(const SegRows sg, const float4* __restrict__ tab, double rdr,
                                                         int nr) {
-  __shared__ float4 offs[NOFF_PAD];
   extern __shared__ float4 lds_raw[];
   float4* tab32 = lds_raw;
-  stage_table(tab32, tab, nr + 1);
+  stage_table(tab32, tab, nr + 2);
   const Sched s = sched(sg.first[sg.n]);
-  int cur = -1;
   for (long ch = s.first; ch < s.end; ch += s.stride) {
     const int k = seg_of(sg.first, sg.n, ch);
-    if (k != cur) {
-      __syncthreads();  // every lane is done with the previous model's offsets
-      stage_offsets(offs, sg.a[k].cs);
-      __syncthreads();
-      cur = k;
-    }
-    density_chunk<TPR, true, NT, U>(sg.a[k], ch - sg.first[k], offs, tab32, nullptr, rdr, nr, sg.out[k]);
+    density_chunk<TPR, true, NT, U>(sg.a[k], ch - sg.first[k], tab32, nullptr, rdr, nr, sg.out[k]);
   }
 }
 
 template <int TPR, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_force_seg(const SegRows sg, EamForceArgs q) {
-  __shared__ float4 offs[NOFF_PAD];
   extern __shared__ float4 lds_raw[];
   float4* tab32 = lds_raw;
-  stage_table(tab32, static_cast<const float4*>(q.tab), 2 * (q.nr + 1));
+  stage_halves(tab32, static_cast<const float4*>(q.tab), q.nr + 2);
   const Sched s = sched(sg.first[sg.n]);
-  int cur = -1;
   for (long ch = s.first; ch < s.end; ch += s.stride) {
     const int k = seg_of(sg.first, sg.n, ch);
-    if (k != cur) {
-      __syncthreads();
-      stage_offsets(offs, sg.a[k].cs);
-      __syncthreads();
-      cur = k;
-    }
-    force_chunk<TPR, true, false, NT, U>(sg.a[k], q, ch - sg.first[k], offs, tab32, nullptr, sg.out[k], sg.gf[k],
+    force_chunk<TPR, true, false, NT, U>(sg.a[k], q, ch - sg.first[k], tab32, nullptr, sg.out[k], sg.gf[k],
                                          sg.P[k], sg.E8[k], nullptr);
   }
 }
@@ -851,21 +858,21 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
       if (MIXED) {
         const float4 pi = a.site32[k];
         const float cut32 = (float)a.cutsq;
+        float fx32 = 0.0f, fy32 = 0.0f, fz32 = 0.0f;
         walk_row<TPR, UNROLL, float4>(
             a.nlist, row0, n, lane, [&](unsigned e) { return a.site32[e & MIX_MASK]; },
-            [&](unsigned ent, const float4& sj, bool valid) {
+            [&](unsigned ent, const float4& sj) {
               float dx, dy, dz;
               mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
-              const float rsq = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
-              const float okf = valid && rsq < cut32 ? 1.0f : 0.0f;
+              const float rsq = dx * dx + dy * dy + dz * dz;
+              const float okf = rsq < cut32 ? 1.0f : 0.0f;  // beyond rc (or the dummy): nothing
               const float r2inv = __builtin_amdgcn_rcpf(rsq);
               const float r6inv = r2inv * r2inv * r2inv;
               const float forcelj = r6inv * (l1 * r6inv - l2);
               const float fpair = okf * (forcelj * r2inv);
-              const double fp = (double)fpair;
-              fx = __builtin_fma((double)dx, fp, fx);
-              fy = __builtin_fma((double)dy, fp, fy);
-              fz = __builtin_fma((double)dz, fp, fz);
+              fx32 = __builtin_fmaf(dx, fpair, fx32);
+              fy32 = __builtin_fmaf(dy, fpair, fy32);
+              fz32 = __builtin_fmaf(dz, fpair, fz32);
               if (EV) {
                 ev += (double)(okf * (r6inv * (l3 * r6inv - l4)));
                 v0 += (double)(dx * dx * fpair);
@@ -875,19 +882,25 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
                 v4 += (double)(dx * dz * fpair);
                 v5 += (double)(dy * dz * fpair);
               }
+            },
+            [&]() {
+              fx += (double)fx32;
+              fy += (double)fy32;
+              fz += (double)fz32;
+              fx32 = fy32 = fz32 = 0.0f;
             });
       } else {
         const double4 pi = a.site[k];
         walk_row<TPR, UNROLL, double4>(
             a.nlist, row0, n, lane, [&](unsigned e) { return a.site[e & SITE_MASK]; },
-            [&](unsigned ent, const double4& sj, bool valid) {
+            [&](unsigned ent, const double4& sj) {
               double shx, shy, shz;
               img_shift(ent >> IMG_SHIFT, a.L, shx, shy, shz);
               const double delx = pi.x - (sj.x + shx);
               const double dely = pi.y - (sj.y + shy);
               const double delz = pi.z - (sj.z + shz);
-              const double rsq = fmax(delx * delx + dely * dely + delz * delz, 1e-200);
-              const double okf = valid && rsq < a.cutsq ? 1.0 : 0.0;
+              const double rsq = delx * delx + dely * dely + delz * delz;
+              const double okf = rsq < a.cutsq ? 1.0 : 0.0;
               const double r2inv = 1.0 / rsq;
               const double r6inv = r2inv * r2inv * r2inv;
               const double forcelj = r6inv * (q.lj1 * r6inv - q.lj2);
@@ -904,7 +917,8 @@ __global__ __launch_bounds__(PB) void k_lj_force(PairArgs a, LjArgs q, double* _
                 v4 += delx * delz * fpair;
                 v5 += dely * delz * fpair;
               }
-            });
+            },
+            []() {});
       }
     }
     fx = row_sum<TPR>(fx);
@@ -1014,7 +1028,7 @@ static int resident_grid(Ctx& c, K kernel, int nt, size_t lds, long nchunks) {
 template <int TPR, bool MIXED, int NT, int U>
 static void run_density(Ctx& c, Gate g) {
   const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
-  const size_t lds_rho = knot * (c.nr + 1);
+  const size_t lds_rho = knot * (c.nr + 2);
   const PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
   const void* tab_rho = MIXED ? (const void*)c.tab32r.get() : (const void*)c.tab64.get();
   ScopedTimer tm(c, "eam_density");
@@ -1030,7 +1044,7 @@ static void run_force(Ctx& c, int eflag, int vflag, Gate g) {
   double* gf = c.gf.get();
   double* er = ev ? c.erow.get() : nullptr;
   const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
-  const size_t lds_force = 2 * knot * (c.nr + 1);
+  const size_t lds_force = 2 * knot * (c.nr + 2);
   const PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
   EamForceArgs q;
   q.tab = MIXED ? (const void*)c.tab32f.get() : (const void*)c.tab64.get();
@@ -1062,7 +1076,6 @@ template <bool MIXED, int NTD, int U>
 static void run_density_tpr(Ctx& c, Gate g) {
   switch (c.tpr) {
     case 4: run_density<4, MIXED, NTD, U>(c, g); break;
-    case 16: run_density<16, MIXED, NTD, U>(c, g); break;
     default: run_density<8, MIXED, NTD, U>(c, g); break;
   }
 }
@@ -1084,7 +1097,6 @@ static void run_eam(Ctx& c, int eflag, int vflag, Gate g) {
   }
   switch (c.tpr_force) {
     case 8: run_force<8, MIXED, NT, U>(c, eflag, vflag, g); break;
-    case 16: run_force<16, MIXED, NT, U>(c, eflag, vflag, g); break;
     default: run_force<4, MIXED, NT, U>(c, eflag, vflag, g); break;
   }
 }
@@ -1126,7 +1138,6 @@ static void launch_pair(Ctx& c, int eflag, int vflag, Gate g) {
   } else {
     switch (c.tpr_force) {
       case 8: run_lj<8>(c, eflag, vflag, g); break;
-      case 16: run_lj<16>(c, eflag, vflag, g); break;
       default: run_lj<4>(c, eflag, vflag, g); break;
     }
   }
@@ -1192,7 +1203,7 @@ void pair_passes_segmented(Ctx* const* cs, int n, hipStream_t stream) {
   fr.first[n] = cf;
   es.first[n] = cb;
   dr.n = fr.n = es.n = n;
-  const size_t lds_rho = sizeof(float4) * (c0.nr + 1);
+  const size_t lds_rho = sizeof(float4) * (c0.nr + 2);
   auto kd = k_eam_density_seg<8, 512, 4>;
   kd<<<resident_grid(c0, kd, 512, lds_rho, cd), 512, lds_rho, stream>>>(
       dr, reinterpret_cast<const float4*>(c0.tab32r.get()), c0.rdr, c0.nr);
@@ -1205,7 +1216,7 @@ void pair_passes_segmented(Ctx* const* cs, int n, hipStream_t stream) {
   q.rhomax = c0.rhomax;
   q.nr = c0.nr;
   q.nrho = c0.nrho;
-  const size_t lds_force = 2 * sizeof(float4) * (c0.nr + 1);
+  const size_t lds_force = 2 * sizeof(float4) * (c0.nr + 2);
   auto kf = k_eam_force_seg<4, 1024, 4>;
   kf<<<resident_grid(c0, kf, 1024, lds_force, cf), 1024, lds_force, stream>>>(fr, q);
   CAC_HIP(hipGetLastError());
