@@ -83,6 +83,7 @@ def lib():
             "cac_hip_set_nodal_force_style": [vp, i],
             "cac_hip_neighbor_settings": [vp, d, i, i, i, i],
             "cac_hip_neigh_build": [vp],
+            "cac_hip_pbc": [vp],
             "cac_hip_neigh_decide": [vp, C.POINTER(i)],
             "cac_hip_neigh_info": [vp, C.POINTER(i), C.POINTER(C.c_long), C.POINTER(i), C.POINTER(i)],
             "cac_hip_neigh_export": [vp, C.POINTER(C.c_long), vp, vp, vp, vp, vp],
@@ -196,6 +197,11 @@ class Model:
 
     def build_neighbor(self):
         self._chk(self.L.cac_hip_neigh_build(self.h))
+
+    def pbc(self):
+        """Domain::pbc on the device model, as write_data runs it; the lists
+        are stale until the next setup() or build_neighbor()."""
+        self._chk(self.L.cac_hip_pbc(self.h))
 
     def compute(self, eflag=1, vflag=1):
         self._chk(self.L.cac_hip_compute(self.h, eflag, vflag))

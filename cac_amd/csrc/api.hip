@@ -757,6 +757,19 @@ int cac_hip_neighbor_settings(cac_hip_ctx* h, double skin, int every, int delay,
   });
 }
 
+// Domain::pbc (domain.cpp:526-738) on the device model, as write_data runs it
+// before writing (write_data.cpp:120-131); the lists are stale afterwards
+int cac_hip_pbc(cac_hip_ctx* h) {
+  return guarded(h, [&](Ctx& c) -> int {
+    int rc = require_model(c);
+    if (rc) return rc;
+    CAC_HIP(hipSetDevice(c.device));
+    wrap_model(c);
+    c.built = false;
+    return CAC_HIP_OK;
+  });
+}
+
 int cac_hip_neigh_build(cac_hip_ctx* h) {
   return guarded(h, [&](Ctx& c) -> int {
     int rc = require_model(c);

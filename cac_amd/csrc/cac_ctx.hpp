@@ -324,6 +324,7 @@ struct Ctx {
 // ---------------------------------------------------------------- launchers
 // neigh.hip
 void launch_pbc(Ctx& c);
+void wrap_model(Ctx& c);  // centres + pbc now, with the non-finite check
 void check_deformation(Ctx& c, bool first);
 void build_neighbor(Ctx& c);  // pbc + sites + cells + full lists
 int check_distance(Ctx& c, bool outer, int* inner);  // 1 when any point moved more than skin/2

@@ -555,6 +555,18 @@ static void read_flags(Ctx& c) {
   read_device_ints(c, reinterpret_cast<const int*>(c.d_flags.get()), 8, reinterpret_cast<int*>(c.h_flags));
 }
 
+// Domain::pbc on the model alone, as write_data calls it before writing
+// (write_data.cpp:120-131): element centres from their nodes, then the wrap.
+// Points move by box lengths, so the lists no longer match the positions: the
+// caller marks them stale and the next setup builds them again.
+void wrap_model(Ctx& c) {
+  CAC_HIP(hipMemsetAsync(c.d_flags.get(), 0, sizeof(ErrorFlags), c.stream));
+  if (c.E) update_centers(c);
+  launch_pbc(c);
+  read_flags(c);
+  if (c.h_flags->nan & 1) throw HipError{hipErrorInvalidValue, "Non-numeric positions - simulation unstable"};
+}
+
 void build_neighbor(Ctx& c) {
   ScopedTimer tm(c, "neigh_build");
   c.ago = 0;

@@ -2,6 +2,7 @@
 // device context, Pair / Neighbor / FixNVE / Verlet mirrors.
 #include "cac_host.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -163,6 +164,80 @@ void read_data(const std::string& path, System& sys) {
   if (ne && !el.ncell[0]) bad(path, "elements but no Element Types section");
 }
 
+// ------------------------------------------------------------------ write_data (write_data.cpp:188-600)
+
+void write_data_file(const std::string& path, const System& sys, long ntimestep, bool velocities) {
+  FILE* fp = std::fopen(path.c_str(), "w");
+  if (!fp) throw Error(CAC_HIP_EARG, "Cannot open data file " + path);
+  const Atoms& at = sys.atom;
+  const Elements& el = sys.element;
+  std::fprintf(fp, "CAC data file via write_data, timestep = %ld\n\n", ntimestep);
+  if (at.n) std::fprintf(fp, "%d atoms\n", at.n);
+  if (el.n) std::fprintf(fp, "%d elements\n", el.n);
+  std::fprintf(fp, "%d atom types\n", (int)sys.mass.size());
+  if (el.n) std::fprintf(fp, "%d element types\n", 1);
+  std::fprintf(fp, "\n");
+  const char* ax[3] = {"x", "y", "z"};
+  for (int d = 0; d < 3; d++)
+    std::fprintf(fp, "%-1.16e %-1.16e %slo %shi\n", sys.boxlo[d], sys.boxhi[d], ax[d], ax[d]);
+  if (!sys.mass.empty()) {
+    std::fprintf(fp, "\nMasses\n\n");
+    for (size_t t = 0; t < sys.mass.size(); t++) std::fprintf(fp, "%d %g\n", (int)t + 1, sys.mass[t]);
+  }
+  if (el.n) {
+    // one Hex type, one atom per unit cell, 2x2x2 Gauss cells (fewer along
+    // an axis of one unit cell), one point per cell
+    std::fprintf(fp, "\nElement Types\n\n");
+    std::fprintf(fp, "%d %s %d %d %d %d %d %d %d %d %d %d\n", 1, "Hex", 1, el.ncell[0], el.ncell[1], el.ncell[2],
+                 std::min(2, el.ncell[0]), std::min(2, el.ncell[1]), std::min(2, el.ncell[2]), 1, 1, 1);
+  }
+  if (at.n) {
+    std::fprintf(fp, "\nAtoms\n\n");
+    for (int i = 0; i < at.n; i++) {
+      const double* x = &at.x[3 * (size_t)i];
+      std::fprintf(fp, "%d %d %-1.16e %-1.16e %-1.16e\n", at.tag[i], at.type[i], x[0], x[1], x[2]);
+    }
+    if (velocities) {
+      std::fprintf(fp, "\nVelocities\n\n");
+      for (int i = 0; i < at.n; i++) {
+        const double* v = &at.v[3 * (size_t)i];
+        std::fprintf(fp, "%d %-1.16e %-1.16e %-1.16e\n", at.tag[i], v[0], v[1], v[2]);
+      }
+    }
+  }
+  if (el.n) {
+    std::fprintf(fp, "\nElements\n\n");
+    for (int e = 0; e < el.n; e++) {
+      double c[3] = {0, 0, 0};
+      for (int k = 0; k < 8; k++)
+        for (int d = 0; d < 3; d++) c[d] += el.nodex[((size_t)e * 8 + k) * 3 + d];
+      std::fprintf(fp, "%d %d %-1.16e %-1.16e %-1.16e\n", el.tag[e], el.etype[e], c[0] / 8, c[1] / 8, c[2] / 8);
+    }
+    std::fprintf(fp, "\nNodes\n\n");
+    for (int e = 0; e < el.n; e++)
+      for (int k = 0; k < 8; k++) {
+        const double* x = &el.nodex[((size_t)e * 8 + k) * 3];
+        std::fprintf(fp, "%d %d %d %d %-1.16e %-1.16e %-1.16e\n", el.tag[e], k + 1, 1, el.ctype[e], x[0], x[1], x[2]);
+      }
+    if (velocities) {
+      std::fprintf(fp, "\nNode Velocities\n\n");
+      for (int e = 0; e < el.n; e++)
+        for (int k = 0; k < 8; k++) {
+          const double* v = &el.nodev[((size_t)e * 8 + k) * 3];
+          std::fprintf(fp, "%d %d %d %-1.16e %-1.16e %-1.16e\n", el.tag[e], k + 1, 1, v[0], v[1], v[2]);
+        }
+    }
+  }
+  const bool ok = std::ferror(fp) == 0;
+  if (std::fclose(fp) != 0 || !ok) throw Error(CAC_HIP_EARG, "write_data " + path + ": write failed");
+}
+
+void write_data(const std::string& path, Device& dev, long ntimestep, bool velocities) {
+  dev.pbc();
+  dev.download_state();
+  write_data_file(path, dev.system(), ntimestep, velocities);
+}
+
 // ------------------------------------------------------------------ device context
 
 Device::Device(System& sys, int device) : sys_(sys) {
@@ -201,6 +276,8 @@ void Device::download_state() {
   if (sys_.element.n)
     check(cac_hip_download_elements(ctx_, sys_.element.nodex.data(), sys_.element.nodev.data(), nullptr, nullptr));
 }
+
+void Device::pbc() { check(cac_hip_pbc(ctx_)); }
 
 void Device::download_forces() {
   check(cac_hip_download_forces(ctx_, sys_.atom.n ? sys_.atom.f.data() : nullptr,

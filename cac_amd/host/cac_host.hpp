@@ -64,6 +64,23 @@ struct System {
 // Entries are placed by id, so sections may list them in any order.
 void read_data(const std::string& path, System& sys);
 
+// write_data (write_data.cpp:188-600) of a model held on the host: header
+// (write_data.cpp:262-283), Masses (:290-294, "%d %g" as there), Element
+// Types (element_vec_cac.cpp:1283-1292), Atoms (atom_vec_atomic.cpp:588-594),
+// Velocities (atom_vec.cpp:101-106), Elements (element_vec_cac.cpp:1177-1182,
+// the centre as the mean of the nodes), Nodes (:1203-1212) and Node
+// Velocities (:1256-1264), in the model's order.  Coordinates and velocities
+// print as %-1.16e (17 significant digits), so read_data reads every double
+// back exactly.  velocities = false is the command's novelocity keyword.
+void write_data_file(const std::string& path, const System& sys, long ntimestep, bool velocities = true);
+
+class Device;
+// the write_data command on the device model: Domain::pbc on the device, as
+// WriteData::command runs it (write_data.cpp:120-131), then the state
+// downloaded into the System and written.  The lists are stale afterwards:
+// the next run's setup rebuilds them, as the reference's does.
+void write_data(const std::string& path, Device& dev, long ntimestep, bool velocities = true);
+
 // The context and the uploaded model.  precision: CAC_HIP_FP64 or CAC_HIP_MIXED.
 class Device {
  public:
@@ -77,6 +94,7 @@ class Device {
   void upload();                       // model arrays host -> device (invalidates lists)
   void sync_positions();               // atom->x, element->nodex host -> device
   void download_state();               // x, v (and nodal x, v) device -> host
+  void pbc();                          // Domain::pbc on the device model (lists stale after)
   void download_forces();              // f, nodef, gaussf device -> host
 
  private:

@@ -7,11 +7,14 @@
 //           [--skin S] [--every N] [--delay N] [--check yes|no] [--dt DT]
 //           [--steps N] [--thermo N] [--mixed] [--loop host|device]
 //           [--dump "ID all custom N file ..."]... [--neighbor-debug "file [atom ID | gauss ID IB IG]"]
-//   cac_run DATA --parse-only [--dump ...]   counts and checksums of the data file
+//           [--write-data FILE [--then M FILE2]]
+//   cac_run DATA --parse-only [--dump ...] [--write-data FILE]   counts and checksums of the data file
 //
 // --dump takes the arguments of the reference's dump command and writes a
 // snapshot at step 0 and every N steps; --neighbor-debug those of
 // write_dump_neighbor_debug, written after the first neighbor build.
+// --write-data is a write_data command after the run; --then M FILE2 a second
+// run command of M steps (with its own setup) and a write_data after it.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -31,7 +34,8 @@ static int usage() {
                "               [--every N] [--delay N] [--check yes|no] [--dt DT] [--steps N] [--thermo N]\n"
                "               [--mixed] [--loop host|device] [--dump \"ID all custom N file ...\"]...\n"
                "               [--neighbor-debug \"file [atom ID | gauss ID IBASIS IGCELL]\"]\n"
-               "       cac_run DATA --parse-only [--dump ...]\n");
+               "               [--write-data FILE [--then M FILE2]]\n"
+               "       cac_run DATA --parse-only [--dump ...] [--write-data FILE]\n");
   return 2;
 }
 
@@ -52,7 +56,8 @@ int main(int argc, char** argv) {
   if (argc < 3) return usage();
   std::vector<std::string> dump_args;
   std::string ndebug_args;
-  std::string data = argv[1], eam, units = "metal", loop = "device";
+  std::string data = argv[1], eam, units = "metal", loop = "device", wdata, wdata2;
+  long then_steps = -1;
   double eps = 1, sigma = 1, cut = 2.5, skin = -1, dt = -1;
   int every = 1, delay = 10, check = 1, steps = 100, thermo = 10;
   bool lj = false, mixed = false, parse_only = false;
@@ -76,6 +81,8 @@ int main(int argc, char** argv) {
     else if (a == "--parse-only") parse_only = true;
     else if (a == "--dump") { need(1); dump_args.push_back(argv[++i]); }
     else if (a == "--neighbor-debug") { need(1); ndebug_args = argv[++i]; }
+    else if (a == "--write-data") { need(1); wdata = argv[++i]; }
+    else if (a == "--then") { need(2); then_steps = std::atol(argv[++i]); wdata2 = argv[++i]; }
     else return usage();
   }
   try {
@@ -87,8 +94,10 @@ int main(int argc, char** argv) {
       Args a(d);
       dumps.emplace_back(new DumpCustom(a.n(), a.v()));
     }
+    if (then_steps >= 0 && wdata.empty()) return usage();
     if (parse_only) {
       for (auto& d : dumps) d->write(sys, 0);
+      if (!wdata.empty()) write_data_file(wdata, sys, 0);
       double sx = 0, sv = 0, sn = 0, snv = 0;
       for (double q : sys.atom.x) sx += q;
       for (double q : sys.atom.v) sv += q;
@@ -172,6 +181,16 @@ int main(int argc, char** argv) {
     std::printf("Step PotEng KinEng TotEng Press\n");
     for (const auto& r : rows) std::printf("%ld %.15g %.15g %.15g %.15g\n", r.step, r.pe, r.ke, r.etotal, r.press);
     std::printf("Neighbor list builds = %d\nDangerous builds = %d\n", neigh.ncalls(), neigh.ndanger());
+    if (!wdata.empty()) write_data(wdata, dev, verlet.ntimestep);
+    if (then_steps >= 0) {
+      // the next run command: Verlet::setup (a neighbor build) and its steps
+      verlet.setup();
+      if (then_steps > 0) {
+        if (loop == "host") verlet.run((int)then_steps, 0);
+        else verlet.run_resident((int)then_steps, dt, 0);
+      }
+      write_data(wdata2, dev, verlet.ntimestep);
+    }
   } catch (const Error& e) {
     std::fprintf(stderr, "ERROR: %s\n", e.what());
     return 1;

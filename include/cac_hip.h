@@ -139,6 +139,14 @@ int cac_hip_thermo_noalloc(cac_hip_ctx *ctx, double *pe, double *ke, double *tem
 int cac_hip_sync_positions(cac_hip_ctx *ctx, const double *x, const double *nodex);
 int cac_hip_download_forces(cac_hip_ctx *ctx, double *f, double *nodef, double *gaussf);
 
+/* Domain::pbc (domain.cpp:526-738) on the device model, as WriteData::command
+ * runs it before writing (write_data.cpp:120-131): atoms wrapped one by one,
+ * elements rigidly by their centre.  Points move by box lengths, so the lists
+ * are stale afterwards: cac_hip_run and cac_hip_neigh_decide return
+ * CAC_HIP_ESTATE until cac_hip_setup (the next run's Verlet::setup) or
+ * cac_hip_neigh_build rebuilds them. */
+int cac_hip_pbc(cac_hip_ctx *ctx);
+
 /* ---- host <-> device ---- */
 int cac_hip_download_atoms(cac_hip_ctx *ctx, double *x, double *v, double *f);
 int cac_hip_download_elements(cac_hip_ctx *ctx, double *nodex, double *nodev, double *nodef, double *gaussf);

@@ -86,6 +86,42 @@ def test_calls_out_of_order_and_bad_arguments():
         L.cac_hip_destroy(h)
 
 
+def test_pbc_wraps_and_leaves_the_lists_stale():
+    """cac_hip_pbc (Domain::pbc as write_data runs it): an atom and an element
+    pushed past a periodic face come back inside the box, shifted by a box
+    length; a run or a neighbor decision before the next setup is
+    CAC_HIP_ESTATE; setup rebuilds and the model runs on."""
+    import cac_amd
+    d = small_cac()
+    L = d.boxhi - d.boxlo
+    a = int(np.argmax(d.x[:, 0]))
+    d.x[a, 0] = d.boxhi[0] + 0.25  # past +x
+    e = 0
+    d.nodex[e, :, 2] -= d.nodex[e, :, 2].mean() - (d.boxlo[2] - 0.5)  # centre below -z
+    g = cac_amd.Model(d, "eam", skin=2.0, eam_tables=TAB)
+    g.pbc()
+    x = g.atoms()[0]
+    nx = g.elements()["nodex"]
+    assert x[a, 0] == (d.boxhi[0] + 0.25) - L[0]
+    # rigid shift by the wrapped centre minus the node mean: L up to rounding
+    assert np.allclose(nx[e, :, 2], d.nodex[e, :, 2] + L[2], rtol=0, atol=1e-12)
+    assert np.array_equal(nx[e, :, :2], d.nodex[e, :, :2])
+    assert np.all((x >= d.boxlo) & (x < d.boxhi))
+    cen = nx.mean(axis=1)
+    assert np.all((cen >= d.boxlo - 1e-12) & (cen < d.boxhi + 1e-12))
+    g.close()
+    g = cac_amd.Model(small_cac(), "eam", skin=2.0, eam_tables=TAB)
+    g.setup()
+    g.pbc()
+    with pytest.raises(cac_amd.CacHipError) as ei:
+        g.run(1, 0.001)
+    assert ei.value.code == 7
+    g.setup()
+    th = g.run(5, 0.001, thermo_every=5)
+    assert np.all(np.isfinite(th))
+    g.close()
+
+
 def test_capacity_grows_without_changing_lists():
     """The first build sizes rows from the mean site density.  An LJ slab that
     fills a quarter of its box has rows about four times that estimate: the

@@ -3,6 +3,7 @@ Pair / Neighbor / FixNVE / Verlet mirrors driving the device through the C ABI
 (GPU tests), checked against the oracle."""
 
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -61,6 +62,78 @@ def test_read_data_rejects_other_gauss_layouts(tmp_path):
     p.write_text(txt)
     r = run(p, "--parse-only", check=False)
     assert r.returncode == 1 and "2 2 2 Gauss cells" in r.stderr
+
+
+WRITE_DATA_DECKS = {
+    "cac": lambda: decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=300.0, seed=1),
+    "atoms": lambda: decks.lj_block(n=4, jitter=0.05, T=0.5, seed=2),
+    "elements": lambda: decks.cac_eam(box_cells=(2, 2, 2), core_cells=0, ncell=5, jitter=0.02, T=100.0, seed=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(WRITE_DATA_DECKS))
+def test_write_data_round_trip(tmp_path, name):
+    """write_data of the host model (write_data.cpp:188-600): the header and
+    section lines as the reference prints them, every coordinate and velocity
+    at %-1.16e, so read_data reads each double back exactly; writing the
+    re-read model again gives the same bytes."""
+    d = WRITE_DATA_DECKS[name]()
+    a, b, c = tmp_path / "a.data", tmp_path / "b.data", tmp_path / "c.data"
+    decks.write_data(str(a), d)
+    run(a, "--parse-only", "--write-data", b)
+    e = decks.read_data(str(b))
+    for key in ("x", "v", "nodex", "nodev", "boxlo", "boxhi", "masses", "atype", "etype", "ctype"):
+        assert np.array_equal(np.asarray(getattr(e, key)), np.asarray(getattr(d, key))), key
+    lines = b.read_text().splitlines()
+    assert lines[0] == "CAC data file via write_data, timestep = 0" and lines[1] == ""
+    assert ("%d atoms" % d.natoms in lines) == (d.natoms > 0)
+    assert ("%d elements" % d.nelements in lines) == (d.nelements > 0)
+    assert ("1 element types" in lines) == (d.nelements > 0)
+    assert lines[lines.index("Masses") + 2] == "1 %g" % d.masses[0]
+    num = r"-?\d\.\d{16}e[+-]\d{2}"
+    assert re.fullmatch(rf"{num} {num} xlo xhi", [ln for ln in lines if ln.endswith("xlo xhi")][0])
+    if d.nelements:
+        nc = d.ncell
+        assert lines[lines.index("Element Types") + 2] == f"1 Hex 1 {nc[0]} {nc[1]} {nc[2]} 2 2 2 1 1 1"
+        assert re.fullmatch(rf"1 1 1 1 {num} {num} {num}", lines[lines.index("Nodes") + 2])
+        assert re.fullmatch(rf"1 1 1 {num} {num} {num}", lines[lines.index("Node Velocities") + 2])
+        # Elements: the centre of each element's nodes
+        cen = np.array([[float(v) for v in ln.split()[2:]] for ln in
+                        lines[lines.index("Elements") + 2:lines.index("Elements") + 2 + d.nelements]])
+        assert np.allclose(cen, d.nodex.mean(axis=1), rtol=0, atol=1e-12)
+    if d.natoms:
+        assert re.fullmatch(rf"1 1 {num} {num} {num}", lines[lines.index("Atoms") + 2])
+        assert re.fullmatch(rf"1 {num} {num} {num}", lines[lines.index("Velocities") + 2])
+    run(b, "--parse-only", "--write-data", c)
+    assert c.read_bytes() == b.read_bytes()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mixed", [False, True])
+def test_write_data_restart_continues_bitwise(tmp_path, mixed):
+    """A checkpoint from device state: 40 steps, write_data (Domain::pbc on the
+    device, then the state written), and a second run of 40 steps in the same
+    process; then read_data of the checkpoint in a new process and 40 steps.
+    Both end states, written by write_data, agree byte for byte (positions and
+    velocities of atoms and nodes), in FP64 and in mixed precision.  The
+    checkpoint holds every point inside the box and differs from the start."""
+    d = decks.cac_eam(box_cells=(3, 3, 3), core_cells=8, ncell=3, jitter=0.03, T=600.0, seed=31)
+    data, setfl = tmp_path / "deck.data", tmp_path / "cu.setfl"
+    ck, fa, fb = tmp_path / "ck.data", tmp_path / "a.data", tmp_path / "b.data"
+    decks.write_data(str(data), d)
+    decks.write_setfl(str(setfl), TAB)
+    prec = ["--mixed"] if mixed else []
+    run(data, "--eam", setfl, "--skin", 1.0, "--steps", 40, *prec, "--write-data", ck, "--then", 40, fa)
+    run(ck, "--eam", setfl, "--skin", 1.0, "--steps", 40, *prec, "--write-data", fb)
+    la, lb = fa.read_text().splitlines(), fb.read_text().splitlines()
+    assert la[0].endswith("timestep = 80") and lb[0].endswith("timestep = 40")
+    assert la[1:] == lb[1:]
+    c = decks.read_data(str(ck))
+    L = c.boxhi - c.boxlo
+    assert np.all((c.x >= c.boxlo) & (c.x < c.boxhi))
+    cen = c.nodex.mean(axis=1)
+    assert np.all((cen >= c.boxlo - 1e-9 * L) & (cen < c.boxhi + 1e-9 * L))
+    assert not np.array_equal(c.x, d.x) and not np.array_equal(c.nodev, d.nodev)
 
 
 @pytest.mark.gpu
