@@ -402,11 +402,14 @@ __device__ __forceinline__ double reference_rsq(const double* __restrict__ x, in
 // BL lanes per row scan the 5x5x5 block of cells around the row's cell (cell
 // edge > (rc+skin)/2, periodic) and keep every site image within rc + skin
 // except the row's own site.  Sites are sorted by cell, so the five cells of
-// one x strip are one contiguous site range (two across the periodic seam):
-// the lanes read it coalesced, and a ballot + prefix per lane group appends
-// the kept sites in scan order (z, y, x cell order, site order inside a
-// cell), so the list is deterministic.  Only an overflowing row touches the
-// shared status words.
+// one x strip are one contiguous site range (two or three across the periodic
+// seams): the lanes read it coalesced, and a ballot + prefix per lane group
+// appends the kept sites in scan order (z, y, x cell order, site order inside
+// a cell), so the list is deterministic.  A box of two to four cells along an
+// axis (rc + skin <= L < 2.5 (rc + skin)) meets a cell more than once in the
+// block, each time as another image, as the reference's ghost shell holds
+// several images of a site when the box is short.  Only an overflowing row
+// touches the shared status words.
 constexpr int BL = 8;
 
 //
@@ -444,7 +447,6 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
   unsigned* row = nlist + (size_t)r * cap;
   const int nx = g.nc[0];
   const int xlo = ic[0] - 2, xhi = ic[0] + 2;
-  const int nseg = (xlo < 0 || xhi >= nx) ? 2 : 1;
   int n = 0;
   for (int dz = -2; dz <= 2; dz++) {
     int cz = ic[2] + dz, tz = 0;
@@ -457,15 +459,22 @@ __global__ __launch_bounds__(256) void k_build_lists(const double4* __restrict__
       else if (cy >= g.nc[1]) { cy -= g.nc[1]; ty = 1; }
       const double shy = (double)ty * g.L[1];
       const long cbase = ((long)cz * g.nc[1] + cy) * nx;
-      for (int sg = 0; sg < nseg; sg++) {
-        // the strip's cells in dx order: the part across the seam first when it lies at -x
-        int a = xlo, b = xhi, tx = 0;
-        if (xlo < 0) {
-          if (sg == 0) { a = xlo + nx; b = nx - 1; tx = -1; }
-          else { a = 0; b = xhi; }
-        } else if (xhi >= nx) {
-          if (sg == 0) { a = xlo; b = nx - 1; }
-          else { a = 0; b = xhi - nx; tx = 1; }
+      for (int tx = -1; tx <= 1; tx++) {
+        // the strip's cells in dx order: the part across the -x seam, the part
+        // inside the box, the part across the +x seam (a box of fewer than
+        // five cells holds all three, and a cell may come twice, as two images)
+        int a, b;
+        if (tx < 0) {
+          if (xlo >= 0) continue;
+          a = xlo + nx;
+          b = nx - 1;
+        } else if (tx > 0) {
+          if (xhi < nx) continue;
+          a = 0;
+          b = xhi - nx;
+        } else {
+          a = xlo > 0 ? xlo : 0;
+          b = xhi < nx ? xhi : nx - 1;
         }
         const double shx = (double)tx * g.L[0];
         const unsigned tcode = (unsigned)((tx + 1) + 3 * (ty + 1) + 9 * (tz + 1));
@@ -584,7 +593,7 @@ void build_neighbor(Ctx& c) {
   // cell grid: edge > (rc + skin)/2 so a 5x5x5 block covers the cutoff
   for (int d = 0; d < 3; d++) {
     int n = (int)floor(c.L[d] / (0.5 * c.cutneigh * (1.0 + 1e-9)));
-    if (n < 5) throw HipError{hipErrorInvalidValue, "box shorter than 2.5 (rc + skin) in some dimension"};
+    if (n < 2) throw HipError{hipErrorInvalidValue, "box shorter than rc + skin in some dimension"};
     c.nc[d] = n;
     c.cinv[d] = (double)n / c.L[d];
   }

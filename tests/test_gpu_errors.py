@@ -56,13 +56,14 @@ def test_element_deformed_twice_its_size_is_an_error():
     g.close()
 
 
-def test_box_too_short_for_the_cutoff_is_an_error():
+def test_box_shorter_than_the_cutoff_is_an_error():
+    """A box shorter than rc + skin would need images beyond the nearest."""
     import cac_amd
-    d = decks.lj_block(n=2, jitter=0.0, T=0.0)  # 3.2 sigma box, rc + skin = 2.8
+    d = decks.lj_block(n=1, jitter=0.0, T=0.0)  # 1.59 sigma box, rc + skin = 2.8
     g = cac_amd.Model(d, "lj", skin=0.3)
     with pytest.raises(cac_amd.CacHipError) as ei:
         g.setup()
-    assert ei.value.code == 1 and "box shorter" in str(ei.value)
+    assert ei.value.code == 1 and "box shorter than rc + skin" in str(ei.value)
     g.close()
 
 
