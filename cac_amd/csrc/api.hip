@@ -31,10 +31,12 @@ Ctx::~Ctx() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+// Timing events record with a device-scope release: a system-scope one (the
+// default) adds its cache write-back to the interval it closes.
 static hipEvent_t take_event(Ctx& c) {
   if (c.ev_used == c.ev_pool.size()) {
     hipEvent_t e = nullptr;
-    CAC_HIP(hipEventCreate(&e));
+    CAC_HIP(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
     c.ev_pool.push_back(e);
   }
   return c.ev_pool[c.ev_used++];

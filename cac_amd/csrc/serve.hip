@@ -586,7 +586,8 @@ int cac_hip_server_create(cac_hip_server** out, int device, int slots, double hb
     Slot sl;
     int rc = cac_hip_create(&sl.h, device, CAC_HIP_UNITS_METAL);
     for (int m = 0; m < MARKS && !rc; m++)
-      if (hipEventCreate(&sl.marks[m]) != hipSuccess) rc = CAC_HIP_EHIP;
+      // device-scope release: the mark times the step, not a system-scope write-back
+      if (hipEventCreateWithFlags(&sl.marks[m], hipEventReleaseToDevice) != hipSuccess) rc = CAC_HIP_EHIP;
     if (rc) {
       s->slots.push_back(sl);
       for (Slot& x : s->slots) {
