@@ -40,17 +40,35 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 WORKLOADS = {
-    # name: (deck builder, description)
+    # name: (deck builder, description, pair style)
     "cac_eam_x8": (lambda d: d.cac_eam(box_cells=(14, 16, 16), core_cells=351, ncell=9, jitter=0.02, T=300.0,
                                        seed=2024),
-                   "255,879 atoms + 13,985 Hex(9^3) elements, EAM, mixed vs FP64"),
+                   "255,879 atoms + 13,985 Hex(9^3) elements, EAM, mixed vs FP64", "eam"),
     "cac_eam": (lambda d: d.cac_eam(box_cells=(7, 8, 8), core_cells=44, ncell=9, jitter=0.02, T=300.0, seed=2024),
-                "32,076 atoms + 1,748 Hex(9^3) elements, EAM, mixed vs FP64"),
+                "32,076 atoms + 1,748 Hex(9^3) elements, EAM, mixed vs FP64", "eam"),
     "cac_dislocation": (lambda d: d.cac_dislocation(),
                         "2,000,493 atoms around an edge dislocation dipole + 40,097 Hex(11^3) elements, "
-                        "55.4 M represented sites, EAM, mixed vs FP64"),
+                        "55.4 M represented sites, EAM, mixed vs FP64", "eam"),
+    "lj_atoms": (lambda d: d.lj_block(n=20, a=1.5874, jitter=0.01, T=0.1, seed=12345),
+                 "32,000-atom FCC LJ block (20^3 cells, a = 1.5874 sigma, rc = 2.5 sigma), LJ units, mixed vs FP64",
+                 "lj"),
 }
-SKIN, DT = 2.0, 0.001
+SKIN, DT = 2.0, 0.001  # metal units: the EAM decks
+# neighbor skin and time step of each pair style's decks (update.cpp:112-184 defaults)
+PAIR_UNITS = {"eam": (SKIN, DT), "lj": (0.3, 0.005)}
+# the workload's pair style, skin and time step (set by main)
+WL = {"pair": "eam", "skin": SKIN, "dt": DT}
+
+
+def progress(msg):
+    """One line on stderr per finished phase (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def model(cac_amd, deck, tab, precision):
+    """A model of the workload deck in the workload's pair style."""
+    return cac_amd.Model(deck, WL["pair"], skin=WL["skin"], eam_tables=tab if WL["pair"] == "eam" else None,
+                         precision=precision)
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "eam_force_mixed_cac_eam_x8_pmc.md")
 VALU_SUMMARY = os.path.join(ROOT, "profiles", "pair_passes_mixed_cac_eam_x8_valu.md")
@@ -79,14 +97,14 @@ def cpu_baseline(deck, tab, nsteps, barrier=None):
     the same deck: set-up (list build + first force), then nsteps NVE steps
     timed; with a barrier, every worker starts its timed steps together."""
     from oracle import oracle as O
-    o = O.Oracle(deck, "eam", skin=SKIN, eam_tables=tab)
+    o = O.Oracle(deck, WL["pair"], skin=WL["skin"], eam_tables=tab if WL["pair"] == "eam" else None)
     t0 = time.perf_counter()
     o.setup(eflag=0, vflag=0)
     t_setup = time.perf_counter() - t0
     if barrier is not None:
         barrier.wait()
     t0 = time.perf_counter()
-    o.run(nsteps, DT)
+    o.run(nsteps, WL["dt"])
     t_run = time.perf_counter() - t0
     o.close()
     return t_setup, t_run
@@ -156,7 +174,7 @@ def handover_run(cac_amd, deck, tab, steps=50, warmup=5):
     nx = np.array(deck.nodex, dtype=np.float64).reshape(-1, 8, 3)
     out = {}
     for name, prec in (("mixed", cac_amd.MIXED), ("fp64", cac_amd.FP64)):
-        m = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=prec)
+        m = model(cac_amd, deck, tab, prec)
         m.setup()
         for _ in range(warmup):
             m.sync_positions(x, nx)
@@ -242,14 +260,14 @@ def per_step_force_error(cac_amd, deck, tab, steps, sample=50):
     whose |f_i| is at least 1 % of rms |f| (points near force balance have no
     meaningful relative error)."""
     import numpy as np
-    m64 = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.FP64)
-    mmx = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.MIXED)
+    m64 = model(cac_amd, deck, tab, cac_amd.FP64)
+    mmx = model(cac_amd, deck, tab, cac_amd.MIXED)
     m64.setup()
     mmx.setup()
     at = set(np.linspace(1, steps, min(sample, steps)).round().astype(int).tolist())
     rows = []
     for k in range(1, steps + 1):
-        m64.run(1, DT, thermo=False)
+        m64.run(1, WL["dt"], thermo=False)
         if k not in at:
             continue
         x = m64.atoms()[0]
@@ -313,11 +331,12 @@ def upkeep_costs(m, reps=5):
 
 
 def force_bytes(P, inner_mean, live_sites):
-    """Algorithmic HBM bytes of one mixed EAM force launch.
+    """Algorithmic HBM bytes of one mixed force launch (EAM or LJ).
 
     Per row: its site index, point index and inner-list length (12 B) and the
     FP64 force it writes (24 B); per inner-list entry its 4-byte word; per
-    site the inner lists reference, its FP32 record (x, y, z, F') read once.
+    site the inner lists reference, its 16-byte FP32 record (x, y, z and F'
+    for EAM) read once.
     """
     return P * (36.0 + 4.0 * inner_mean) + 16.0 * live_sites
 
@@ -395,7 +414,10 @@ def main():
     from cac_amd import decks
 
     rates = [float(r) for r in args.serve_rates.split(",") if r]
-    build_deck, desc = WORKLOADS[args.workload]
+    build_deck, desc, pair = WORKLOADS[args.workload]
+    WL.update(pair=pair, skin=PAIR_UNITS[pair][0], dt=PAIR_UNITS[pair][1])
+    dt = WL["dt"]
+    force_name = "eam_force" if pair == "eam" else "lj_force"
     tab = decks.cu_eam_tables()
     if args.serve_only:
         print(json.dumps({"serving": serve_run(cac_amd, decks, tab, args.serve_replicas, args.serve_steps,
@@ -408,12 +430,12 @@ def main():
         return
 
     if args.profile_run:
-        m = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.MIXED)
+        m = model(cac_amd, deck, tab, cac_amd.MIXED)
         if args.inner_skin >= 0:
             m.set_inner_skin(args.inner_skin)
         m.setup()
         print(f"profile-run: {args.workload} set up", flush=True)
-        m.run(args.steps, DT, thermo=False)
+        m.run(args.steps, dt, thermo=False)
         m.sync()
         _, nbar_in, _, live = m.inner_stats()
         print(f"profile-run: {args.steps} mixed steps done; inner mean {nbar_in:.2f}, live sites {live}, "
@@ -425,10 +447,13 @@ def main():
     # CPU baselines first: the worker processes are forked before this
     # process touches the GPU
     cpu = None
+    progress(f"{args.workload}: {deck.natoms} atoms + {deck.nelements} elements, {P} atom-eq")
     if not args.no_cpu_baseline:
         ncpu, workers = host_cores()
         t_setup, t_run = cpu_baseline(deck, tab, args.cpu_steps)
+        progress(f"CPU baseline, one thread: {P * args.cpu_steps / t_run:.3g} atom-eq steps/s")
         agg, t_agg, _ = cpu_baseline_all_cores(deck, tab, args.cpu_steps_all, workers)
+        progress(f"CPU baseline, {workers} processes: {agg:.3g} atom-eq steps/s")
         cpu = {"value": P * args.cpu_steps / t_run, "unit": "atom-eq steps/s", "cores": 1, "kind": "port",
                "sample": f"oracle (CPU restatement, one thread) running {args.cpu_steps} NVE steps of this deck "
                          f"after its setup; setup (neighbor build + first force, {t_setup:.1f} s) excluded",
@@ -439,10 +464,12 @@ def main():
 
     # mixed-vs-FP64 force error per step along an FP64 trajectory (own models)
     step_err = per_step_force_error(cac_amd, deck, tab, args.steps) if args.error_steps else None
+    if step_err:
+        progress("per-step force error done")
 
     # FP64 and mixed models on the same deck
-    m64 = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.FP64)
-    mmx = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.MIXED)
+    m64 = model(cac_amd, deck, tab, cac_amd.FP64)
+    mmx = model(cac_amd, deck, tab, cac_amd.MIXED)
     if args.inner_skin >= 0:
         m64.set_inner_skin(args.inner_skin)
         mmx.set_inner_skin(args.inner_skin)
@@ -460,35 +487,38 @@ def main():
     pp = df[keep] / fmag[keep]
     info = mmx.neigh_info()
     nbar = info["entries"] / info["rows"]
+    progress(f"set-up of both models done; mixed vs FP64 force error {max_rel_err:.3g}")
 
     b0, ib0 = mmx.neigh_stats()[0], mmx.inner_stats()[0]
     # the force pass is timed live over the timed steps: one hipEvent pair per
     # launch on the model's stream, read back after the timed region
-    mmx.run(args.warmup, DT, thermo=False)
+    mmx.run(args.warmup, dt, thermo=False)
     mmx.sync()
     b1, ib1 = mmx.neigh_stats()[0], mmx.inner_stats()[0]
-    t_mx = timed_run(mmx, args.steps, 0, DT, force_timing=True)
-    force_us, force_launches = mmx.kernel_time("eam_force")
+    t_mx = timed_run(mmx, args.steps, 0, dt, force_timing=True)
+    force_us, force_launches = mmx.kernel_time(force_name)
     mmx.enable_timing(False)
     nsteps_mx = args.steps + args.warmup
     rebuilds = (mmx.neigh_stats()[0] - b0) / nsteps_mx
     prunes = (mmx.inner_stats()[0] - ib0) / nsteps_mx
     n_build_timed = mmx.neigh_stats()[0] - b1
     n_prune_timed = mmx.inner_stats()[0] - ib1
-    t_64 = timed_run(m64, args.steps, args.warmup, DT)
+    t_64 = timed_run(m64, args.steps, args.warmup, dt)
     value = P * args.steps / t_mx
     value64 = P * args.steps / t_64
+    progress(f"timed runs: mixed {value:.4g}, FP64 {value64:.4g} atom-eq steps/s")
 
     # per-kernel durations from hipEvents on the context's stream (a separate
     # short run, so the events do not perturb the timed one)
-    names = ("eam_density", "eam_embed", "eam_force", "site_refresh", "nodef", "nve", "neigh_build", "inner_build")
+    pair_names = ("eam_density", "eam_embed", "eam_force") if pair == "eam" else ("lj_force",)
+    names = pair_names + ("site_refresh", "nodef", "nve", "neigh_build", "inner_build")
     mmx.enable_timing(True)
-    mmx.run(args.profile_steps, DT, thermo=False)
+    mmx.run(args.profile_steps, dt, thermo=False)
     kt = {k: mmx.kernel_time(k) for k in names}
     mmx.enable_timing(False)
     m64.enable_timing(True)
-    m64.run(args.profile_steps, DT, thermo=False)
-    kt64 = {k: m64.kernel_time(k) for k in ("eam_density", "eam_embed", "eam_force")}
+    m64.run(args.profile_steps, dt, thermo=False)
+    kt64 = {k: m64.kernel_time(k) for k in pair_names}
     m64.enable_timing(False)
     _, nbar_in, delta, live = mmx.inner_stats()
 
@@ -498,12 +528,13 @@ def main():
     # after 20 warm-up steps, from the same set-up) shows; the timed window's
     # step without upkeep is its time less the upkeep it held.
     c_prune, c_build = upkeep_costs(mmx)
-    mwin = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=cac_amd.MIXED)
+    progress(f"upkeep at rest: re-prune {1000 * c_prune:.3f} ms, rebuild {1000 * c_build:.3f} ms")
+    mwin = model(cac_amd, deck, tab, cac_amd.MIXED)
     if args.inner_skin >= 0:
         mwin.set_inner_skin(args.inner_skin)
     mwin.setup()
     wb0, wp0 = mwin.neigh_stats()[0], mwin.inner_stats()[0]
-    mwin.run(220, DT, thermo=False)
+    mwin.run(220, dt, thermo=False)
     mwin.sync()
     win_builds, win_prunes = mwin.neigh_stats()[0] - wb0, mwin.inner_stats()[0] - wp0
     mwin.close()
@@ -519,7 +550,7 @@ def main():
         "note": "value is the timed --steps window (the contract); this is the steady-state rate that "
                 "spreads the measured upkeep costs over the fixed window's intervals"}
 
-    # dominant kernel: the mixed EAM force pass, timed over the timed steps
+    # dominant kernel: the mixed force pass, timed over the timed steps
     bytes_force = force_bytes(P, nbar_in, live)
     achieved = bytes_force / (force_us * 1e-6) / 1e9
     tr = traffic_from_summary() if args.workload == "cac_eam_x8" else None
@@ -536,10 +567,11 @@ def main():
                 "source": os.path.relpath(VALU_SUMMARY, ROOT)}
     # the whole step against HBM: 224 + 8 N (N = mean full-list entries) bytes
     # per atom-eq step for EAM (state streamed by the two list passes and the
-    # integrator, one 8-byte list entry per neighbor and pass); list upkeep
-    # separately: a rebuild writes and re-reads the outer list and the site
-    # records, a re-prune reads the outer list and writes the inner one
-    b_step = 224.0 + 8.0 * nbar
+    # integrator, one 8-byte list entry per neighbor and pass), 168 + 4 N for
+    # LJ (one list pass); list upkeep separately: a rebuild writes and re-reads
+    # the outer list and the site records, a re-prune reads the outer list and
+    # writes the inner one
+    b_step = 224.0 + 8.0 * nbar if pair == "eam" else 168.0 + 4.0 * nbar
     b_rebuild = P * 8.0 * nbar + 32.0 * info["sites"] + 8.0 * deck.represented_sites
     b_prune = P * (4.0 * nbar + 4.0 * nbar_in + 32.0) + 16.0 * info["sites"]
     step_roof = {"bytes_per_atom_eq_step": b_step, "achieved_GBs": value * b_step / 1e9,
@@ -564,14 +596,15 @@ def main():
             drift[key] = {"deck": label, "steps": args.drift_steps, "fp64": d64, "mixed": dmx,
                           "ratio_mixed_over_fp64": abs(dmx["drift_eV_per_atom_ps"]) /
                           max(abs(d64["drift_eV_per_atom_ps"]), 1e-300)}
+            progress(f"NVE drift on {key} done")
         # the benchmark deck itself over its stable window: both precisions
         # follow the CAC equations' flutter growth, so the energy change is
         # that growth, not integration error
         w = {}
         for name, prec in (("fp64", cac_amd.FP64), ("mixed", cac_amd.MIXED)):
-            mm = cac_amd.Model(deck, "eam", skin=SKIN, eam_tables=tab, precision=prec)
+            mm = model(cac_amd, deck, tab, prec)
             mm.setup()
-            th = mm.run(200, DT, thermo_every=50)
+            th = mm.run(200, dt, thermo_every=50)
             mm.close()
             w[name] = {"e0": float(th[0, 3]), "e_200": float(th[-1, 3]),
                        "rel_change": float((th[-1, 3] - th[0, 3]) / abs(th[0, 3]))}
@@ -581,6 +614,7 @@ def main():
     serving = None
     if args.serve_replicas > 0:
         serving = serve_run(cac_amd, decks, tab, args.serve_replicas, args.serve_steps, rates, args.serve_slots)
+        progress("serving done")
 
     line = {
         # the served p50/p99 step times are the "serving" object of the same line
@@ -595,10 +629,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 pair terms / fp64 positions and sums",
-        "data": "synthetic (seeded FCC Cu CAC deck, analytic EAM tabulated as setfl)",
+        "data": ("synthetic (seeded FCC Cu CAC deck, analytic EAM tabulated as setfl)" if pair == "eam"
+                 else "synthetic (seeded FCC LJ block)"),
         "config": {"workload": args.workload, "description": desc, "atoms": deck.natoms,
                    "elements": deck.nelements, "atom_equivalents": P, "represented_sites": deck.represented_sites,
-                   "skin": SKIN, "dt_ps": DT, "neighbor_mean_full": round(nbar, 2), "sites": info["sites"],
+                   "pair_style": pair, "skin": WL["skin"], "dt": dt,
+                   "units": "metal (A, ps)" if pair == "eam" else "lj (sigma, tau)",
+                   "neighbor_mean_full": round(nbar, 2), "sites": info["sites"],
                    "inner_delta": delta, "inner_mean": round(nbar_in, 2), "inner_live_sites": live,
                    "outer_builds_per_step": round(rebuilds, 4), "inner_prunes_per_step": round(prunes, 4)},
         "amortized": amortized,
@@ -612,7 +649,7 @@ def main():
                                 "per_step": step_err},
         "kernels_us": {k: round(v[0], 3) for k, v in kt.items() if v[1]},
         "kernels_us_fp64": {k: round(v[0], 3) for k, v in kt64.items() if v[1]},
-        "roofline": {"bound": "hbm", "kernel": "eam_force (mixed)", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": f"{force_name} (mixed)", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "kernel_us": force_us, "kernel_launches": force_launches,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tr["bytes_per_launch"] if tr else None,
