@@ -305,8 +305,9 @@ class Model:
                                                  nf.ctypes.data if E else None, gf.ctypes.data if E else None))
         return f, nf, gf
 
-    def full_sets(self):
-        """Full lists as sorted rows (point, kind, owner, ucell, ix, iy, iz)."""
+    def full_sets(self, sort=True):
+        """Full lists as rows (point, kind, owner, ucell, ix, iy, iz), sorted
+        unless sort is False."""
         n = C.c_long()
         self._chk(self.L.cac_hip_neigh_export(self.h, C.byref(n), None, None, None, None, None))
         m = n.value
@@ -314,6 +315,8 @@ class Model:
         img = np.zeros((m, 3), dtype=np.int32)
         self._chk(self.L.cac_hip_neigh_export(self.h, C.byref(n), _p(row), _p(kind), _p(owner), _p(ucell), _p(img)))
         arr = np.stack([row, kind, owner, ucell, img[:, 0], img[:, 1], img[:, 2]], axis=1).astype(np.int64)
+        if not sort:
+            return arr
         order = np.lexsort(tuple(arr[:, c] for c in range(6, -1, -1)))
         return arr[order]
 

@@ -172,8 +172,7 @@ def test_cac_eam_x8_workload_forces():
     and nodes, densities, energy and virial within 1e-12 of the oracle's, the
     full-list size equal to that of the oracle's half list expanded (a pair
     whose site is an atom or a Gauss point counts twice), and mixed-precision
-    forces within 1e-5 of FP64.  (The sorted sets themselves are compared at the
-    46,060-point size: sorting 49 M rows takes minutes on the host.)"""
+    forces within 1e-5 of FP64.  The sets themselves: the next test."""
     import cac_amd
     deck = decks.cac_eam(box_cells=(14, 16, 16), core_cells=351, ncell=9, jitter=0.02, T=300.0, seed=2024)
     o, g = setup_pair(deck, "eam", 2.0)
@@ -194,6 +193,26 @@ def test_cac_eam_x8_workload_forces():
     f32 = np.concatenate([gm.atoms()[2], gm.elements()["gaussf"].reshape(-1, 3)])
     assert rel(f32, f64) < 1e-5, rel(f32, f64)
     gm.close()
+
+
+def test_cac_eam_x8_workload_sets():
+    """The benchmark workload's sorted per-point sets at full size (49.3 M
+    full-list entries, 367,759 points) equal the oracle's half list expanded,
+    compared as sorted packed int64 keys (one per row)."""
+    from oracle import oracle as O
+    deck = decks.cac_eam(box_cells=(14, 16, 16), core_cells=351, ncell=9, jitter=0.02, T=300.0, seed=2024)
+    o = oracle_model(deck, "eam", 2.0)
+    o.build_neighbor()
+    p, k, ow, u, img = o.half_list()
+    o.close()
+    ref = O.canonical_full_keys(deck, p, k, ow, u, img, brute.corner_ucells(deck.ncell))
+    del p, k, ow, u, img
+    g = gpu_model(deck, "eam", 2.0)
+    g.build_neighbor()
+    got = np.sort(O.pack_set_rows(deck, g.full_sets(sort=False)))
+    g.close()
+    assert got.size == ref.size and got.size > 4.9e7
+    assert np.array_equal(got, ref)
 
 
 def test_lj_full_workload_momentum():
