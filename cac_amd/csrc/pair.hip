@@ -367,17 +367,21 @@ __device__ __forceinline__ void stage_table_strided(V* lds, const V* __restrict_
 }
 
 // ------------------------------------------------------------------ row walker
-// The list build pads every inner row to a multiple of 16 entries with an
-// entry naming a dummy site far from every point, so every word a lane loads
-// up to the row's length rounded up to 2 TPR lies inside its row: no index
-// clamp and no row-end mask.  An entry beyond the cutoff (a dummy among them)
-// evaluates the all-zero spline knot nr + 1 appended to the tables, so it
-// contributes an exact zero.
+// The list build pads every inner row to a multiple of ROW_PAD (16) entries
+// with an entry naming a dummy site far from every point, and the passes walk
+// the whole padded row: no index clamp and no row-end mask.  An entry beyond
+// the cutoff (a dummy among them) evaluates the all-zero spline knot nr + 1
+// appended to the tables, so it contributes an exact zero.
 //
 // Lane l of a row's TPR lanes takes entries l, l + TPR, ...: full batches of
 // U entries per lane (U list loads, then U site gathers, then U pair terms,
-// so U independent memory chains are in flight), then the padded tail in
-// batches of 2 entries per lane.  `flush` runs after every GROUP entries of a
+// so U independent memory chains are in flight), then what is left of the
+// padded row in batches of 2 entries per lane (none for 4 lanes x 4 entries;
+// one for 8 x 4 and a row of 16 entries past its last full batch).  A whole
+// batch of dummies is cheaper than a short tail: rounding the force pass's
+// rows to 8 entries instead (a tail batch of 2 per lane) made it 4 % slower,
+// and a last batch of 1 per lane, which leaves one memory chain in flight,
+// slower still.  `flush` runs after every GROUP entries of a
 // lane and after its last one (the passes move their FP32 group sums into
 // FP64 sums there); groups follow the lane's entry count, not the batches, so
 // the sums do not depend on U.
@@ -392,7 +396,7 @@ __device__ __forceinline__ void walk_row(const unsigned* __restrict__ base, unsi
   static_assert(ROW_PAD % (UT * TPR) == 0, "tail batches must stay inside the padded row");
   static_assert(U % GROUP == 0, "a full batch holds whole groups");
   const unsigned* __restrict__ rp = base + row0 + lane;
-  const int np = (n + UT * TPR - 1) / (UT * TPR) * (UT * TPR);
+  const int np = (n + ROW_PAD - 1) / ROW_PAD * ROW_PAD;
   const int nfull = np / B;
   for (int b = 0; b < nfull; b++) {
     unsigned w[U];
