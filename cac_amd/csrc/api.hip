@@ -14,6 +14,7 @@
 
 #include "../../include/cac_hip.h"
 #include "cac_ctx.hpp"
+#include "eam_tables.hpp"
 
 using namespace cac;
 
@@ -292,30 +293,6 @@ void hex_shape(double* s, double px, double py, double pz) {  // element.cpp:298
 
 const int HEX_SGN[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                            {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
-
-// PairEAM::interpolate (pair_eam.cpp:586-614), f[1..n] -> spl[(n+1)*7]
-void interpolate(int n, double delta, const std::vector<double>& f, std::vector<double>& spl) {
-  spl.assign((size_t)(n + 1) * 7, 0.0);
-  auto S = [&](int m, int k) -> double& { return spl[(size_t)m * 7 + k]; };
-  for (int m = 1; m <= n; m++) S(m, 6) = f[m];
-  S(1, 5) = S(2, 6) - S(1, 6);
-  S(2, 5) = 0.5 * (S(3, 6) - S(1, 6));
-  S(n - 1, 5) = 0.5 * (S(n, 6) - S(n - 2, 6));
-  S(n, 5) = S(n, 6) - S(n - 1, 6);
-  for (int m = 3; m <= n - 2; m++)
-    S(m, 5) = ((S(m - 2, 6) - S(m + 2, 6)) + 8.0 * (S(m + 1, 6) - S(m - 1, 6))) / 12.0;
-  for (int m = 1; m <= n - 1; m++) {
-    S(m, 4) = 3.0 * (S(m + 1, 6) - S(m, 6)) - 2.0 * S(m, 5) - S(m + 1, 5);
-    S(m, 3) = S(m, 5) + S(m + 1, 5) - 2.0 * (S(m + 1, 6) - S(m, 6));
-  }
-  S(n, 4) = 0.0;
-  S(n, 3) = 0.0;
-  for (int m = 1; m <= n; m++) {
-    S(m, 2) = S(m, 5) / delta;
-    S(m, 1) = 2.0 * S(m, 4) / delta;
-    S(m, 0) = 3.0 * S(m, 3) / delta;
-  }
-}
 
 template <typename T>
 void upload(DBuf<T>& b, const std::vector<T>& h, hipStream_t s) {
@@ -673,22 +650,15 @@ int cac_hip_pair_eam(cac_hip_ctx* h, int nrho, double drho, int nr, double dr, d
     upload(c.rhor_spl, rs, c.stream);
     upload(c.z2r_spl, zs, c.stream);
     // FP32 tables of the mixed path, staged in LDS by the pair kernels: the
-    // density pass reads rhor's value coefficients c3..c6; the force pass
-    // z2r's value coefficients and the derivative coefficients c0..c2 of rhor
-    // and c2 of z2r (z2r's c0, c1 are 3 c3 / dr and 2 c4 / dr, formed in the
-    // kernel), two float4 per knot
-    // one knot more than the reference's table: knot nr + 1 is all zeros, and
-    // the pair passes evaluate it for a list entry beyond the cutoff, so such
-    // an entry contributes an exact zero without a mask multiply
-    std::vector<float> t32r((size_t)(nr + 2) * 4, 0.0f), t32f((size_t)(nr + 2) * 8, 0.0f);
-    for (int m = 0; m <= nr; m++) {
-      for (int q = 0; q < 4; q++) {
-        t32r[(size_t)m * 4 + q] = (float)rs[(size_t)m * 7 + 3 + q];
-        t32f[(size_t)m * 8 + q] = (float)zs[(size_t)m * 7 + 3 + q];
-      }
-      for (int q = 0; q < 3; q++) t32f[(size_t)m * 8 + 4 + q] = (float)rs[(size_t)m * 7 + q];
-      t32f[(size_t)m * 8 + 7] = (float)zs[(size_t)m * 7 + 2];
-    }
+    // density pass reads rhor's value coefficients c3..c6 per knot, the force
+    // pass the planes A, B (and C) of mixed_force_table.  Like the FP64
+    // tables, one knot more than the reference's: knot nr + 1 is all zeros,
+    // and the pair passes evaluate it for a list entry beyond the cutoff, so
+    // such an entry contributes an exact zero without a mask multiply.
+    std::vector<float> t32r((size_t)(nr + 2) * 4, 0.0f), t32f;
+    for (int m = 0; m <= nr; m++)
+      for (int q = 0; q < 4; q++) t32r[(size_t)m * 4 + q] = (float)rs[(size_t)m * 7 + 3 + q];
+    mixed_force_table(nr, dr, rs, zs, t32f);
     std::vector<double> t64((size_t)(nr + 2) * 8, 0.0);
     for (int m = 0; m <= nr; m++)
       for (int q = 0; q < 4; q++) {

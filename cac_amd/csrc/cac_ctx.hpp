@@ -260,13 +260,13 @@ struct Ctx {
   double drho = 0, dr = 0, rdr = 0, rdrho = 0, rhomax = 0, cutmax = 0, cutforcesq = 0;
   DBuf<double> frho_spl, rhor_spl, z2r_spl;  // [(n+1)*7]
   DBuf<float> tab32r;  // mixed density pass, per knot: rhor c3..c6 (FP32)
-  DBuf<float> tab32f;  // mixed force pass, per knot: z2r c3..c6, then rhor c0..c2 and z2r c2 (FP32)
-  DBuf<double> tab64;  // the same in FP64
+  DBuf<float> tab32f;  // mixed force pass: planes A, B, C of nr + 2 float4 (cubics in the knot fraction)
+  DBuf<double> tab64;  // FP64 passes, per knot: rhor c3..c6 and z2r c3..c6 (the reference's coefficients)
   int precision = 0;     // 0 FP64, 1 mixed
   int tpr = 8;           // lanes per neighbor row in the density pass
   int tpr_force = 4;     // lanes per neighbor row in the force passes (EAM, LJ)
   int grid_cap = 4096;   // max workgroups of the persistent pair kernels
-  int pair_shape = 1;    // EAM pass shape: 0/1/2/3 = (512|1024 threads) x (4|8 entries in flight)
+  int pair_shape = 2;    // EAM pass shape: 0/1/2/3 = (512|1024 threads) x (4|8 entries in flight)
   int dens_nt = 512;     // workgroup size of the EAM density pass (512 or 1024)
   int ncu = 256;         // compute units of the device
   std::map<std::pair<const void*, size_t>, int> resident;  // workgroups per CU of a kernel at an LDS size

@@ -319,9 +319,10 @@ __device__ __forceinline__ int knot_r(T r, T rdr, int nr, T& p) {
 // reference forms them for r < (nr - 1) dr.  Past that (the last knot
 // interval, where the tabulated functions of a potential that reaches zero
 // at its cutoff are near zero) the reference clamps to knot nr - 1 at p = 1;
-// here knot nr at its fraction continues the spline linearly, and m is kept
-// at most nr + 1 (the all-zero knot), so a table read stays inside the table.
-// An entry beyond the cutoff takes the zero knot: the caller's `inside`.
+// here knot nr continues it (the density table linearly, the force tables
+// with the clamped value), and m is kept at most nr + 1 (the all-zero knot),
+// so a table read stays inside the table.  An entry beyond the cutoff takes
+// the zero knot: the caller's `inside`.
 __device__ __forceinline__ int knot_r32(float r, float rdr, int nr, bool inside, float& p) {
   const float x = r * rdr + 1.0f;
   p = __builtin_amdgcn_fractf(x);
@@ -609,27 +610,20 @@ __device__ __forceinline__ void force_chunk(const PairArgs& a, const EamForceArg
             float dx, dy, dz;
             mixed_delta(ent, pi, sj, a.cs, dx, dy, dz);
             const float rsq = dx * dx + dy * dy + dz * dz;
-            const float recip = __builtin_amdgcn_rsqf(rsq);
-            const float rr = rsq * recip;
             float p;
             // beyond rc (or the dummy): the zero knot
-            const int m = knot_r32(rr, rdr32, q.nr, rsq < cut32, p);
-            const float4 cz = tab32[m];             // z2r value c3..c6
-            const float4 cd = tab32[q.nr + 2 + m];  // rhor derivative c0..c2, z2r c2
-            const float rhop = (cd.x * p + cd.y) * p + cd.z;
-            // z2 = ((c3 p + c4) p + c5) p + c6 by Horner, and dz2/dp from
-            // the same partial sums; z2' = dz2/dp * rdr
-            const float h1 = cz.x * p + cz.y;
-            const float h2 = h1 * p + cz.z;
-            const float z2 = h2 * p + cz.w;
-            const float dz2 = (cz.x * p + h1) * p + h2;
-            const float phi = z2 * recip;
-            const float phip = (dz2 * rdr32 - phi) * recip;
-            const float fpair = -((pi.w + sj.w) * rhop + phip) * recip;
+            const int m = knot_r32(__builtin_amdgcn_sqrtf(rsq), rdr32, q.nr, rsq < cut32, p);
+            const float4 ca = tab32[m];             // A = -rho'(r) / r
+            const float4 cb = tab32[q.nr + 2 + m];  // B = -phi'(r) / r
+            const float fa = ((ca.x * p + ca.y) * p + ca.z) * p + ca.w;
+            const float fb = ((cb.x * p + cb.y) * p + cb.z) * p + cb.w;
+            const float fpair = __builtin_fmaf(pi.w + sj.w, fa, fb);
             fx32 = __builtin_fmaf(dx, fpair, fx32);
             fy32 = __builtin_fmaf(dy, fpair, fy32);
             fz32 = __builtin_fmaf(dz, fpair, fz32);
             if (EV) {
+              const float4 cc = tab32[2 * (q.nr + 2) + m];  // C = phi(r)
+              const float phi = ((cc.x * p + cc.y) * p + cc.z) * p + cc.w;
               ephi += (double)phi;
               v0 += (double)(dx * dx * fpair);
               v1 += (double)(dy * dy * fpair);
@@ -754,7 +748,7 @@ __global__ __launch_bounds__(NT) void k_eam_force(PairArgs a, EamForceArgs q, do
   float4* tab32 = lds_raw;
   double4* lds64 = reinterpret_cast<double4*>(lds_raw);
   if (MIXED) {
-    stage_halves(tab32, static_cast<const float4*>(q.tab), q.nr + 2);
+    stage_table(tab32, static_cast<const float4*>(q.tab), (EV ? 3 : 2) * (q.nr + 2));
   } else {
     stage_halves(lds64, static_cast<const double4*>(q.tab), q.nr + 2);
   }
@@ -818,7 +812,7 @@ template <int TPR, int NT, int U>
 __global__ __launch_bounds__(NT) void k_eam_force_seg(const SegRows sg, EamForceArgs q) {
   extern __shared__ float4 lds_raw[];
   float4* tab32 = lds_raw;
-  stage_halves(tab32, static_cast<const float4*>(q.tab), q.nr + 2);
+  stage_table(tab32, static_cast<const float4*>(q.tab), 2 * (q.nr + 2));
   const Sched s = sched(sg.first[sg.n]);
   for (long ch = s.first; ch < s.end; ch += s.stride) {
     const int k = seg_of(sg.first, sg.n, ch);
@@ -1048,7 +1042,8 @@ static void run_force(Ctx& c, int eflag, int vflag, Gate g) {
   double* gf = c.gf.get();
   double* er = ev ? c.erow.get() : nullptr;
   const size_t knot = MIXED ? sizeof(float4) : sizeof(double4);
-  const size_t lds_force = 2 * knot * (c.nr + 2);
+  // mixed: the A and B planes, and C in energy/virial steps
+  const size_t lds_force = (MIXED && ev ? 3 : 2) * knot * (c.nr + 2);
   const PairArgs a = pair_args(c, c.cutforcesq, NT / TPR);
   EamForceArgs q;
   q.tab = MIXED ? (const void*)c.tab32f.get() : (const void*)c.tab64.get();
@@ -1165,7 +1160,7 @@ void pair_passes(Ctx& c, int eflag, int vflag, Gate g) {
 
 bool segmentable(const Ctx& c) {
   return c.precision == 1 && c.pairstyle == 2 && c.tpr == 8 && c.tpr_force == 4 && c.dens_nt == 512 &&
-         c.pair_shape == 1 && !c.timing && c.P > 0;
+         c.pair_shape == 2 && !c.timing && c.P > 0;
 }
 
 int max_segments() { return MAXSEG; }
@@ -1187,7 +1182,7 @@ void pair_passes_segmented(Ctx* const* cs, int n, hipStream_t stream) {
     dr.first[k] = cd;
     cd += dr.a[k].nchunks;
     dr.out[k] = c.rho.get();
-    fr.a[k] = pair_args(c, c.cutforcesq, 1024 / 4);
+    fr.a[k] = pair_args(c, c.cutforcesq, 512 / 4);
     fr.first[k] = cf;
     cf += fr.a[k].nchunks;
     fr.out[k] = c.f.get();
@@ -1208,7 +1203,7 @@ void pair_passes_segmented(Ctx* const* cs, int n, hipStream_t stream) {
   es.first[n] = cb;
   dr.n = fr.n = es.n = n;
   const size_t lds_rho = sizeof(float4) * (c0.nr + 2);
-  auto kd = k_eam_density_seg<8, 512, 4>;
+  auto kd = k_eam_density_seg<8, 512, 8>;
   kd<<<resident_grid(c0, kd, 512, lds_rho, cd), 512, lds_rho, stream>>>(
       dr, reinterpret_cast<const float4*>(c0.tab32r.get()), c0.rdr, c0.nr);
   k_embed_seg<<<cb, 256, 0, stream>>>(es, c0.frho_spl.get(), c0.rdrho, c0.nrho);
@@ -1221,8 +1216,8 @@ void pair_passes_segmented(Ctx* const* cs, int n, hipStream_t stream) {
   q.nr = c0.nr;
   q.nrho = c0.nrho;
   const size_t lds_force = 2 * sizeof(float4) * (c0.nr + 2);
-  auto kf = k_eam_force_seg<4, 1024, 4>;
-  kf<<<resident_grid(c0, kf, 1024, lds_force, cf), 1024, lds_force, stream>>>(fr, q);
+  auto kf = k_eam_force_seg<4, 512, 8>;
+  kf<<<resident_grid(c0, kf, 512, lds_force, cf), 512, lds_force, stream>>>(fr, q);
   CAC_HIP(hipGetLastError());
 }
 
