@@ -377,55 +377,50 @@ __device__ __forceinline__ void stage_table_strided(V* lds, const V* __restrict_
 // Lane l of a row's TPR lanes takes entries l, l + TPR, ...: full batches of
 // U entries per lane (U list loads, then U site gathers, then U pair terms,
 // so U independent memory chains are in flight), then what is left of the
-// padded row in batches of 2 entries per lane (none for 4 lanes x 4 entries;
-// one for 8 x 4 and a row of 16 entries past its last full batch).  A whole
-// batch of dummies is cheaper than a short tail: rounding the force pass's
-// rows to 8 entries instead (a tail batch of 2 per lane) made it 4 % slower,
-// and a last batch of 1 per lane, which leaves one memory chain in flight,
-// slower still.  `flush` runs after every GROUP entries of a
+// padded row in one batch each of U/2, U/4, ... 2 entries per lane, as far
+// as they fit.  Short batches leave few memory chains in flight, so a row is
+// walked to its padding (a batch of dummies costs less than a short tail:
+// rounding rows to 8 entries made the force pass 4 % slower, a last batch
+// of 1 entry per lane slower still), and the remainder takes the largest
+// batches that fit.  `flush` runs after every GROUP entries of a
 // lane and after its last one (the passes move their FP32 group sums into
 // FP64 sums there); groups follow the lane's entry count, not the batches, so
 // the sums do not depend on U.
 constexpr int UNROLL = 4;  // entries per lane in flight in a full batch
 constexpr int GROUP = 4;   // entries of a lane summed in FP32 before the FP64 sum
 
+// One level of a row walk: batches of UB entries per lane while a whole batch
+// fits in the padded row, then the level with half as many, down to 2.
+template <int TPR, int UB, typename Site, typename Fetch, typename Body, typename Flush>
+__device__ __forceinline__ void walk_level(const unsigned* __restrict__ rp, int& j, int np, int& cnt, Fetch fetch,
+                                           Body body, Flush flush) {
+  static_assert(ROW_PAD % (2 * TPR) == 0, "the last level must stay inside the padded row");
+  for (; j + UB * TPR <= np; j += UB * TPR) {
+    unsigned w[UB];
+#pragma unroll
+    for (int u = 0; u < UB; u++) w[u] = rp[j + u * TPR];
+    Site sj[UB];
+#pragma unroll
+    for (int u = 0; u < UB; u++) sj[u] = fetch(w[u]);
+#pragma unroll
+    for (int u = 0; u < UB; u++) {
+      body(w[u], sj[u]);
+      // the levels above this one leave cnt a multiple of GROUP
+      if (UB % GROUP == 0 ? (u + 1) % GROUP == 0 : (cnt + u + 1) % GROUP == 0) flush();
+    }
+    cnt += UB;
+  }
+  if constexpr (UB > 2) walk_level<TPR, UB / 2, Site>(rp, j, np, cnt, fetch, body, flush);
+}
+
 template <int TPR, int U, typename Site, typename Fetch, typename Body, typename Flush>
 __device__ __forceinline__ void walk_row(const unsigned* __restrict__ base, unsigned row0, int n, int lane,
                                          Fetch fetch, Body body, Flush flush) {
-  constexpr int UT = 2;  // entries per lane in a tail batch
-  constexpr int B = U * TPR;
-  static_assert(ROW_PAD % (UT * TPR) == 0, "tail batches must stay inside the padded row");
-  static_assert(U % GROUP == 0, "a full batch holds whole groups");
+  static_assert(U % GROUP == 0 && (U & (U - 1)) == 0, "a full batch holds whole groups; levels halve it");
   const unsigned* __restrict__ rp = base + row0 + lane;
   const int np = (n + ROW_PAD - 1) / ROW_PAD * ROW_PAD;
-  const int nfull = np / B;
-  for (int b = 0; b < nfull; b++) {
-    unsigned w[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) w[u] = rp[b * B + u * TPR];
-    Site sj[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) sj[u] = fetch(w[u]);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      body(w[u], sj[u]);
-      if ((u + 1) % GROUP == 0) flush();
-    }
-  }
-  static_assert(GROUP % UT == 0, "tail batches fill whole groups");
-  int cnt = 0;  // the lane's entries since its last full batch
-  for (int j = nfull * B; j < np; j += UT * TPR) {
-    unsigned e[UT];
-    Site sj[UT];
-#pragma unroll
-    for (int u = 0; u < UT; u++) e[u] = rp[j + u * TPR];
-#pragma unroll
-    for (int u = 0; u < UT; u++) sj[u] = fetch(e[u]);
-#pragma unroll
-    for (int u = 0; u < UT; u++) body(e[u], sj[u]);
-    cnt += UT;
-    if (cnt % GROUP == 0) flush();
-  }
+  int j = 0, cnt = 0;
+  walk_level<TPR, U, Site>(rp, j, np, cnt, fetch, body, flush);
   if (cnt % GROUP) flush();
 }
 
