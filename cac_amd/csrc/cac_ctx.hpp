@@ -351,6 +351,20 @@ void compact_live(Ctx& c);  // dense live-site records from the full ones (after
 void export_lists(Ctx& c, std::vector<int>& row, std::vector<int>& kind, std::vector<int>& owner,
                   std::vector<int>& ucell, std::vector<int>& img);
 void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total);
+// Dense live-site arrays written by scan_live for every flagged site k at its
+// index i: descriptors, and the record (FP32 from site32f, or FP64 from site).
+struct LiveOut {
+  const int *sdesc, *simg, *scell;
+  const double4* site;
+  const float4* site32f;
+  int *lsdesc, *lsimg, *lscell;
+  float4* site32;
+  double4* site_live;
+  int mixed;
+};
+// idx = exclusive scan of the 0/1 flags, *d_total (device) = their sum, and the
+// flagged sites compacted into lo's dense arrays, in one pass after the sums
+void scan_live(Ctx& c, const int* flag, int* idx, int n, int* d_total, const LiveOut& lo);
 // stable sort of the candidate sites by cell: sort_keys / sort_vals, cell_start
 void sort_sites_by_cell(Ctx& c, const int* tcell, const unsigned long long* tval, int S);
 void site_positions(Ctx& c);  // FP64 records of every site from its descriptor (list build)

@@ -891,12 +891,9 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
   }
 }
 
-// live-site count, dense index of every row's site, and the inner-list
-// entries re-pointed at the dense records (16 lanes per row; a lane reads all
-// its entries, then all their dense indices, then writes)
-__global__ void k_live_count(const int* __restrict__ flag, const int* __restrict__ idx, int S, int* __restrict__ n) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *n = S ? idx[S - 1] + flag[S - 1] : 0;
-}
+// dense index of every row's site, and the inner-list entries re-pointed at
+// the dense records (16 lanes per row; a lane reads all its entries, then all
+// their dense indices, then writes)
 
 // Each row is then padded to a multiple of ROW_PAD entries (the pair passes
 // read whole batches) with an entry naming the dummy record at index nlive,
@@ -973,8 +970,6 @@ void build_inner(Ctx& c) {
     k_prune<false><<<blocks, 256, 0, c.stream>>>(c.site.get(), c.site32f.get(), c.row_site.get(), c.row_point.get(),
                                                   c.nnum.get(), nl, c.P, c.cap, g, c.inner_cutsq, c.innum.get(), il,
                                                   c.icap, c.x.get(), c.xhold_in.get(), c.live_flag.get());
-  scan_exclusive(c, c.live_flag.get(), c.live_idx.get(), c.S, nullptr);
-  k_live_count<<<1, 64, 0, c.stream>>>(c.live_flag.get(), c.live_idx.get(), c.S, c.d_nlive.get());
   compact_live(c);
   // the dummy entry: no cell offset (mixed) / no image shift (FP64)
   const unsigned pad_code = mixed ? (62u << MIX_SHIFT) : (13u << IMG_SHIFT);

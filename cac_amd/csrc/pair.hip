@@ -209,30 +209,6 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, U
   }
 }
 
-// dense live records from the full ones, after a prune marked the live sites
-template <bool MIXED>
-__global__ __launch_bounds__(256) void k_live_compact(const int* __restrict__ flag, const int* __restrict__ idx, int S,
-                                                      Frame fr, const int* __restrict__ sdesc,
-                                                      const int* __restrict__ simg, const int* __restrict__ scell,
-                                                      const double4* __restrict__ site,
-                                                      const float4* __restrict__ site32f, int* __restrict__ lsdesc,
-                                                      int* __restrict__ lsimg, int* __restrict__ lscell,
-                                                      float4* __restrict__ site32, double4* __restrict__ site_live) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S || !flag[k]) return;
-  const int i = idx[k];
-  lsdesc[i] = sdesc[k];
-  lsimg[i] = simg[k];
-  lscell[i] = scell[k];
-  if (MIXED) {
-    const float4 r = site32f[k];
-    site32[i] = make_float4(r.x, r.y, r.z, 0.0f);
-  } else {
-    const double4 y = site[k];
-    site_live[i] = make_double4(y.x, y.y, y.z, 0.0);
-  }
-}
-
 UcellGrid ucell_grid(const Ctx& c) {
   UcellGrid ug;
   for (int d = 0; d < 3; d++) ug.h[d] = c.ncell[d] > 1 ? 2.0 / (c.ncell[d] - 1) : 0.0;
@@ -291,14 +267,15 @@ void site_positions(Ctx& c) {
       nullptr, nullptr, nullptr, nullptr, nullptr, 1, Gate());
 }
 
+// dense live records from the full ones, after a prune flagged the live sites:
+// live_idx, the live count (device) and the dense arrays in one scan
 void compact_live(Ctx& c) {
   reserve_live(c);
-  const Frame fr = frame_of(c);
-  auto kc = c.precision == 1 ? k_live_compact<true> : k_live_compact<false>;
-  kc<<<grid_for(c.S, 256), 256, 0, c.stream>>>(c.live_flag.get(), c.live_idx.get(), c.S, fr, c.sdesc.get(),
-                                               c.simg.get(), c.scell.get(), c.site.get(), c.site32f.get(),
-                                               c.lsdesc.get(), c.lsimg.get(), c.lscell.get(), c.site32.get(),
-                                               c.site_live.get());
+  c.live_idx.reserve(c.S);
+  c.d_nlive.reserve(1);
+  LiveOut lo{c.sdesc.get(), c.simg.get(), c.scell.get(), c.site.get(), c.site32f.get(), c.lsdesc.get(),
+             c.lsimg.get(), c.lscell.get(), c.site32.get(), c.site_live.get(), c.precision == 1 ? 1 : 0};
+  scan_live(c, c.live_flag.get(), c.live_idx.get(), c.S, c.d_nlive.get(), lo);
 }
 
 // ------------------------------------------------------------------ spline helpers

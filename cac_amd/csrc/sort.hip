@@ -15,19 +15,193 @@
 
 namespace cac {
 
-// Exclusive scan of n ints (single-pass, decoupled look-back); *total (host)
-// receives the sum.  Integer sums are exact in any order, so the result is
-// deterministic.
+// ------------------------------------------------------------------ scans
+// Exclusive scan of n small non-negative ints in three launches over tiles of
+// SCAN_TILE elements: the tiles' sums, their exclusive scan in one workgroup
+// (which also leaves the total on the device), then each tile's scan plus its
+// offset.  Integer sums are exact in any order, so the result is
+// deterministic.  The live-site variant also compacts the flagged sites in
+// the last launch, which then reads each tile's flags once for both.
+constexpr int SCAN_NT = 256;
+constexpr int SCAN_IPT = 16;  // consecutive elements per thread
+constexpr int SCAN_TILE = SCAN_NT * SCAN_IPT;
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// exclusive prefix of x over the workgroup's NT threads; tot = their sum
+template <int NT>
+__device__ __forceinline__ int block_excl_scan(int x, int* lds, int& tot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int inc = wave_incl_scan(x);
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  int before = 0;
+  tot = 0;
+  for (int k = 0; k < NT / 64; k++) {
+    const int s = lds[k];
+    before += k < w ? s : 0;
+    tot += s;
+  }
+  __syncthreads();  // lds is written again by the next call
+  return before + inc - x;
+}
+
+__device__ __forceinline__ void load_run(const int* __restrict__ in, long base, long n, int* v) {
+  if (base + SCAN_IPT <= n) {
+    const int4* p = reinterpret_cast<const int4*>(in + base);
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT / 4; q++) {
+      const int4 t = p[q];
+      v[4 * q] = t.x;
+      v[4 * q + 1] = t.y;
+      v[4 * q + 2] = t.z;
+      v[4 * q + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) v[j] = base + j < n ? in[base + j] : 0;
+  }
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_tiles(const int* __restrict__ in, int n, int* __restrict__ tsum) {
+  __shared__ int lds[SCAN_NT / 64];
+  int v[SCAN_IPT];
+  load_run(in, (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_IPT, n, v);
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_IPT; j++) s += v[j];
+  int tot;
+  block_excl_scan<SCAN_NT>(s, lds, tot);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+// one workgroup: the tile sums become exclusive tile offsets in place
+__global__ __launch_bounds__(1024) void k_scan_offsets(int* __restrict__ tsum, int nt, int* __restrict__ total) {
+  __shared__ int lds[1024 / 64];
+  int carry = 0;
+  for (int b = 0; b < nt; b += 1024) {
+    const int i = b + (int)threadIdx.x;
+    const int x = i < nt ? tsum[i] : 0;
+    int tot;
+    const int ex = block_excl_scan<1024>(x, lds, tot);
+    if (i < nt) tsum[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+// MODE 0: plain scan; 1 / 2: live sites with FP32 / FP64 records
+template <int MODE>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const int* __restrict__ in, int n,
+                                                        const int* __restrict__ toff, int* __restrict__ out,
+                                                        LiveOut lo) {
+  __shared__ int lds[SCAN_NT / 64];
+  __shared__ int sidx[MODE ? SCAN_TILE : 1];
+  const long tile0 = (long)blockIdx.x * SCAN_TILE;
+  const long base = tile0 + (long)threadIdx.x * SCAN_IPT;
+  int v[SCAN_IPT];
+  load_run(in, base, n, v);
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_IPT; j++) s += v[j];
+  int tot;
+  int run = toff[blockIdx.x] + block_excl_scan<SCAN_NT>(s, lds, tot);
+  int o[SCAN_IPT];
+#pragma unroll
+  for (int j = 0; j < SCAN_IPT; j++) {
+    o[j] = run;
+    run += v[j];
+  }
+  if (base + SCAN_IPT <= n) {
+    int4* p = reinterpret_cast<int4*>(out + base);
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT / 4; q++) p[q] = make_int4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++)
+      if (base + j < n) out[base + j] = o[j];
+  }
+  if (MODE) {
+    // each flagged site's dense index (-1 when unflagged), staged so that the
+    // copies below run over consecutive sites across the workgroup
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) sidx[threadIdx.x * SCAN_IPT + j] = v[j] ? o[j] : -1;
+    __syncthreads();
+    // batches of CB sites per thread: all loads, then all stores (the output
+    // arrays could alias the inputs as far as the compiler knows, so loads
+    // after a store would wait for it)
+    constexpr int CB = 8;
+    for (int j0 = 0; j0 < SCAN_IPT; j0 += CB) {
+      int ii[CB], dd[CB], mm[CB], cc[CB];
+      float4 r32[CB];
+      double4 r64[CB];
+#pragma unroll
+      for (int u = 0; u < CB; u++) {
+        const int e = (j0 + u) * SCAN_NT + (int)threadIdx.x;
+        const long k = tile0 + e;
+        ii[u] = k < n ? sidx[e] : -1;
+        if (ii[u] >= 0) {
+          dd[u] = lo.sdesc[k];
+          mm[u] = lo.simg[k];
+          cc[u] = lo.scell[k];
+          if (MODE == 1)
+            r32[u] = lo.site32f[k];
+          else
+            r64[u] = lo.site[k];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CB; u++) {
+        const int i = ii[u];
+        if (i < 0) continue;
+        lo.lsdesc[i] = dd[u];
+        lo.lsimg[i] = mm[u];
+        lo.lscell[i] = cc[u];
+        if (MODE == 1)
+          lo.site32[i] = make_float4(r32[u].x, r32[u].y, r32[u].z, 0.0f);
+        else
+          lo.site_live[i] = make_double4(r64[u].x, r64[u].y, r64[u].z, 0.0);
+      }
+    }
+  }
+}
+
+static void scan_launch(Ctx& c, const int* in, int* out, int n, int* d_total, const LiveOut* lo) {
+  const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  c.scan_tmp.reserve((size_t)nt + 1);
+  if (nt > 0) k_scan_tiles<<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get());
+  k_scan_offsets<<<1, 1024, 0, c.stream>>>(c.scan_tmp.get(), nt, d_total);
+  if (nt > 0) {
+    if (!lo)
+      k_scan_apply<0><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, LiveOut{});
+    else if (lo->mixed)
+      k_scan_apply<1><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, *lo);
+    else
+      k_scan_apply<2><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, *lo);
+  }
+  CAC_HIP(hipGetLastError());
+}
+
+// *total (host) receives the sum
 void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total) {
   if (n <= 0) {
     if (total) *total = 0;
     return;
   }
-  size_t tmp = 0;
-  CAC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, c.stream));
-  c.scan_tmp.reserve(tmp / sizeof(int) + 16);
-  CAC_HIP(hipcub::DeviceScan::ExclusiveSum(c.scan_tmp.get(), tmp, in, out, n, c.stream));
+  scan_launch(c, in, out, n, nullptr, nullptr);
   if (total) *total = read_device_sum(c, in + n - 1, out + n - 1);
+}
+
+void scan_live(Ctx& c, const int* flag, int* idx, int n, int* d_total, const LiveOut& lo) {
+  scan_launch(c, flag, idx, n > 0 ? n : 0, d_total, &lo);
 }
 
 // slot i opens the cells after the previous slot's cell up to its own
