@@ -283,7 +283,8 @@ struct Ctx {
   DBuf<int> d_steps;              // [rebuild step, prune step] written by the displacement check
   // live sites: the sites the inner lists read, compacted after every prune
   // into dense arrays in site (cell) order; the inner lists and the rows index them
-  DBuf<int> live_flag, live_idx, lsdesc, lsimg, lscell, row_live, d_nlive;
+  DBuf<int> live_idx, lsdesc, lsimg, lscell, row_live, d_nlive;
+  DBuf<unsigned char> live_flag;  // one byte per site: the 4 MB of flags of a large model stay in L2
   DBuf<double4> site_live;        // FP64 records (x, y, z, F') of the live sites
   DBuf<float4> site32f;           // mixed path: FP32 cell-relative records of every site (for the prune)
   volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
@@ -364,7 +365,7 @@ struct LiveOut {
 };
 // idx = exclusive scan of the 0/1 flags, *d_total (device) = their sum, and the
 // flagged sites compacted into lo's dense arrays, in one pass after the sums
-void scan_live(Ctx& c, const int* flag, int* idx, int n, int* d_total, const LiveOut& lo);
+void scan_live(Ctx& c, const unsigned char* flag, int* idx, int n, int* d_total, const LiveOut& lo);
 // stable sort of the candidate sites by cell: sort_keys / sort_vals, cell_start
 void sort_sites_by_cell(Ctx& c, const int* tcell, const unsigned long long* tval, int S);
 void site_positions(Ctx& c);  // FP64 records of every site from its descriptor (list build)

@@ -794,7 +794,7 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
                                                const int* __restrict__ nnum, const unsigned* __restrict__ nlist,
                                                int R, int cap, Grid g, double cutsq_in, int* __restrict__ innum,
                                                unsigned* __restrict__ ilist, int icap, const double* __restrict__ x,
-                                               double* __restrict__ xhold_in, int* __restrict__ live) {
+                                               double* __restrict__ xhold_in, unsigned char* __restrict__ live) {
   constexpr int PU = MIXED ? 9 : 6;
   // LDS tables of the mixed test: the cell shift of each of the 27 image codes,
   // packed as the cell-index difference it adds, and the 125 cell-offset
@@ -958,7 +958,7 @@ void build_inner(Ctx& c) {
   c.live_idx.reserve(c.S);
   c.row_live.reserve(c.P);
   c.d_nlive.reserve(1);
-  CAC_HIP(hipMemsetAsync(c.live_flag.get(), 0, sizeof(int) * c.S, c.stream));
+  CAC_HIP(hipMemsetAsync(c.live_flag.get(), 0, c.S, c.stream));
   const int blocks = (int)std::min<long>(grid_for(16L * c.P, 256), 8L * c.ncu);
   const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
   unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());

@@ -54,6 +54,19 @@ __device__ __forceinline__ int block_excl_scan(int x, int* lds, int& tot) {
   return before + inc - x;
 }
 
+// the flags of 16 sites as one 16-byte load
+__device__ __forceinline__ void load_run(const unsigned char* __restrict__ in, long base, long n, int* v) {
+  if (base + SCAN_IPT <= n) {
+    const uint4 t = *reinterpret_cast<const uint4*>(in + base);
+    const unsigned w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) v[j] = (int)((w[j >> 2] >> (8 * (j & 3))) & 0xffu);
+  } else {
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) v[j] = base + j < n ? (int)in[base + j] : 0;
+  }
+}
+
 __device__ __forceinline__ void load_run(const int* __restrict__ in, long base, long n, int* v) {
   if (base + SCAN_IPT <= n) {
     const int4* p = reinterpret_cast<const int4*>(in + base);
@@ -71,7 +84,8 @@ __device__ __forceinline__ void load_run(const int* __restrict__ in, long base, 
   }
 }
 
-__global__ __launch_bounds__(SCAN_NT) void k_scan_tiles(const int* __restrict__ in, int n, int* __restrict__ tsum) {
+template <typename T>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_tiles(const T* __restrict__ in, int n, int* __restrict__ tsum) {
   __shared__ int lds[SCAN_NT / 64];
   int v[SCAN_IPT];
   load_run(in, (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_IPT, n, v);
@@ -99,8 +113,8 @@ __global__ __launch_bounds__(1024) void k_scan_offsets(int* __restrict__ tsum, i
 }
 
 // MODE 0: plain scan; 1 / 2: live sites with FP32 / FP64 records
-template <int MODE>
-__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const int* __restrict__ in, int n,
+template <int MODE, typename T>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const T* __restrict__ in, int n,
                                                         const int* __restrict__ toff, int* __restrict__ out,
                                                         LiveOut lo) {
   __shared__ int lds[SCAN_NT / 64];
@@ -174,18 +188,19 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const int* __restrict__ 
   }
 }
 
-static void scan_launch(Ctx& c, const int* in, int* out, int n, int* d_total, const LiveOut* lo) {
+template <typename T>
+static void scan_launch(Ctx& c, const T* in, int* out, int n, int* d_total, const LiveOut* lo) {
   const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
   c.scan_tmp.reserve((size_t)nt + 1);
-  if (nt > 0) k_scan_tiles<<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get());
+  if (nt > 0) k_scan_tiles<T><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get());
   k_scan_offsets<<<1, 1024, 0, c.stream>>>(c.scan_tmp.get(), nt, d_total);
   if (nt > 0) {
     if (!lo)
-      k_scan_apply<0><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, LiveOut{});
+      k_scan_apply<0, T><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, LiveOut{});
     else if (lo->mixed)
-      k_scan_apply<1><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, *lo);
+      k_scan_apply<1, T><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, *lo);
     else
-      k_scan_apply<2><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, *lo);
+      k_scan_apply<2, T><<<nt, SCAN_NT, 0, c.stream>>>(in, n, c.scan_tmp.get(), out, *lo);
   }
   CAC_HIP(hipGetLastError());
 }
@@ -200,7 +215,7 @@ void scan_exclusive(Ctx& c, const int* in, int* out, int n, int* total) {
   if (total) *total = read_device_sum(c, in + n - 1, out + n - 1);
 }
 
-void scan_live(Ctx& c, const int* flag, int* idx, int n, int* d_total, const LiveOut& lo) {
+void scan_live(Ctx& c, const unsigned char* flag, int* idx, int n, int* d_total, const LiveOut& lo) {
   scan_launch(c, flag, idx, n > 0 ? n : 0, d_total, &lo);
 }
 
