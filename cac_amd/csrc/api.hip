@@ -65,17 +65,17 @@ static bool timed_section(const Ctx& c, const char* nm) {
   return c.timing == 2 && (std::strcmp(nm, "eam_force") == 0 || std::strcmp(nm, "lj_force") == 0);
 }
 
-ScopedTimer::ScopedTimer(Ctx& cc, const char* nm) : c(cc), name(nm) {
+ScopedTimer::ScopedTimer(Ctx& cc, const char* nm, bool kernel_bound) : c(cc), name(nm), bound(kernel_bound) {
   if (!timed_section(c, nm)) return;
   if (c.ev_pending.size() >= 512) resolve_timers(c);
   start = take_event(c);
   stop = take_event(c);
-  CAC_HIP(hipEventRecord(start, c.stream));
+  if (!bound) CAC_HIP(hipEventRecord(start, c.stream));
 }
 
 ScopedTimer::~ScopedTimer() {
   if (!start) return;
-  if (hipEventRecord(stop, c.stream) != hipSuccess) return;
+  if (!bound && hipEventRecord(stop, c.stream) != hipSuccess) return;
   c.ev_pending.push_back({name, start, stop});
 }
 

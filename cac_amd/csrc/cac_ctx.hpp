@@ -408,11 +408,17 @@ double* scratch_on(int device);
 void reduce_rows(Ctx& c, const double* data, int n, int ncomp, double* out);
 
 // timing helpers (api.hip)
-struct ScopedTimer {  // GPU time between two events on the context stream; sections may nest
+// GPU time between two events on the context stream; sections may nest.
+// kernel_bound: the section is one kernel launched with hipExtLaunchKernel,
+// which takes the two events and gives them the kernel's own start and end
+// times (a plain event pair also spans the dispatch of the launch and its
+// completion signal); the caller launches with start / stop when they are set.
+struct ScopedTimer {
   Ctx& c;
   const char* name;
+  bool bound = false;
   hipEvent_t start = nullptr, stop = nullptr;
-  ScopedTimer(Ctx& cc, const char* nm);
+  ScopedTimer(Ctx& cc, const char* nm, bool kernel_bound = false);
   ~ScopedTimer();
 };
 // waits for the stream and adds every recorded section to its timer

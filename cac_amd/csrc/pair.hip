@@ -26,6 +26,8 @@
 // Mixed precision: distances come from FP64 positions and are rounded to FP32;
 // r, the spline evaluations (FP32 tables staged in LDS) and the pair term run
 // in FP32; density, force, energy and virial accumulate in FP64.
+#include <hip/hip_ext.h>
+
 #include "cac_ctx.hpp"
 
 namespace cac {
@@ -1029,14 +1031,17 @@ static void run_force(Ctx& c, int eflag, int vflag, Gate g) {
   q.escale_node = c.nodal_weight / 2.0;
   q.vscale_node = c.nodal_weight / 2.0;
   q.peratom = peratom_wanted(eflag, vflag) ? c.peratom.get() : nullptr;
-  ScopedTimer tm(c, "eam_force");
-  if (ev) {
-    auto kf = k_eam_force<TPR, MIXED, true, NT, U>;
-    const int grid = resident_grid(c, kf, NT, lds_force, a.nchunks);
-    kf<<<grid, NT, lds_force, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er, g);
+  ScopedTimer tm(c, "eam_force", true);
+  auto kf = ev ? k_eam_force<TPR, MIXED, true, NT, U> : k_eam_force<TPR, MIXED, false, NT, U>;
+  const int grid = resident_grid(c, kf, NT, lds_force, a.nchunks);
+  if (tm.start) {
+    // timed: the events take the kernel's own start and end
+    double* f = c.f.get();
+    int P = c.P;
+    void* args[] = {(void*)&a, (void*)&q, (void*)&f, (void*)&gf, (void*)&P, (void*)&E8, (void*)&er, (void*)&g};
+    CAC_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(kf), dim3(grid), dim3(NT), args, lds_force, c.stream,
+                               tm.start, tm.stop, 0));
   } else {
-    auto kf = k_eam_force<TPR, MIXED, false, NT, U>;
-    const int grid = resident_grid(c, kf, NT, lds_force, a.nchunks);
     kf<<<grid, NT, lds_force, c.stream>>>(a, q, c.f.get(), gf, c.P, E8, er, g);
   }
 }
