@@ -158,7 +158,8 @@ struct ErrorFlags {  // device-written status words
   int maxcount;      // largest row seen in the last build
   int rebuild;       // check_distance result (outer list)
   int rebuild_in;    // inner list no longer covers rc
-  int pad[2];
+  int rows;          // the sites that are points did not number P rows (an internal fault)
+  int pad;
 };
 
 // Gate of the speculative no-rebuild step: the host queues a whole step
@@ -219,7 +220,7 @@ struct Ctx {
   int nc[3] = {0, 0, 0};
   long ncells = 0;
   double cinv[3] = {0, 0, 0};
-  DBuf<double4> site, site2;
+  DBuf<double4> site;
   DBuf<float4> site32;  // mixed path: (x, y, z) - cell corner, F'(rho)
   DBuf<int> scell, scell2;  // cell of each site (the frame of site32)
   DBuf<int> sdesc, simg, sdesc2, simg2, used, newidx;

@@ -298,11 +298,17 @@ __global__ void k_unpack_sites(const int* __restrict__ key, const unsigned long 
   isrow[k] = desc >= 0;
 }
 
+// rows in site order: rowidx is the exclusive scan of isrow; a count other
+// than P is flagged (read with the list build's status) and no row past P is
+// written
 __global__ void k_rows(const int* __restrict__ sdesc, const int* __restrict__ isrow, const int* __restrict__ rowidx,
-                       int S, int* __restrict__ row_point, int* __restrict__ row_site) {
+                       int S, int P, int* __restrict__ row_point, int* __restrict__ row_site, ErrorFlags* flags) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S || !isrow[k]) return;
+  if (k >= S) return;
+  if (k == S - 1 && rowidx[k] + isrow[k] != P) flags->rows = 1;
+  if (!isrow[k]) return;
   const int r = rowidx[k];
+  if (r >= P) return;
   row_point[r] = sdesc[k];
   row_site[r] = k;
 }
@@ -530,15 +536,15 @@ __global__ __launch_bounds__(256) void k_mark_used(const int* __restrict__ row_s
   for (int i = lane; i < n; i += 16) used[row[i] & SITE_MASK] = 1;
 }
 
+// the descriptors of the kept sites; their records are written by the full
+// refresh that follows the build
 __global__ void k_compact_sites(const int* __restrict__ used, const int* __restrict__ newidx, int S,
-                                const double4* __restrict__ site, const int* __restrict__ sdesc,
-                                const int* __restrict__ simg, const int* __restrict__ scell,
-                                double4* __restrict__ site2, int* __restrict__ sdesc2, int* __restrict__ simg2,
+                                const int* __restrict__ sdesc, const int* __restrict__ simg,
+                                const int* __restrict__ scell, int* __restrict__ sdesc2, int* __restrict__ simg2,
                                 int* __restrict__ scell2) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S || !used[k]) return;
   const int d = newidx[k];
-  site2[d] = site[k];
   sdesc2[d] = sdesc[k];
   simg2[d] = simg[k];
   scell2[d] = scell[k];
@@ -648,11 +654,9 @@ void build_neighbor(Ctx& c) {
   c.S = S;
   site_positions(c);
   c.rowidx.reserve(S);
-  int nrows = 0;
-  scan_exclusive(c, c.isflag.get(), c.rowidx.get(), S, &nrows);
-  if (nrows != P) throw HipError{hipErrorInvalidValue, "row count mismatch"};
-  k_rows<<<grid_for(S, 256), 256, 0, c.stream>>>(c.sdesc.get(), c.isflag.get(), c.rowidx.get(), S,
-                                                 c.row_point.get(), c.row_site.get());
+  scan_exclusive(c, c.isflag.get(), c.rowidx.get(), S, nullptr);
+  k_rows<<<grid_for(S, 256), 256, 0, c.stream>>>(c.sdesc.get(), c.isflag.get(), c.rowidx.get(), S, P,
+                                                 c.row_point.get(), c.row_site.get(), c.d_flags.get());
   order_rows(c);
   c.nvirt = nvirt;
 
@@ -685,6 +689,7 @@ void build_neighbor(Ctx& c) {
     if (need > c.one) throw HipError{hipErrorInvalidValue, "Neighbor list overflow, boost neigh_modify one"};
     c.cap = ((need + 15) / 16) * 16 + 16;
   }
+  if (c.h_flags->rows) throw HipError{hipErrorUnknown, "row count mismatch"};
   if (c.h_flags->nan & 1) throw HipError{hipErrorInvalidValue, "Non-numeric positions - simulation unstable"};
   if (c.h_flags->nan & 2) throw HipError{hipErrorInvalidValue, "an element is larger than the periodic box"};
   if (c.h_flags->deform) throw HipError{hipErrorInvalidValue, "Simulation unstable! an element deformed by more than 2.0 times"};
@@ -699,17 +704,14 @@ void build_neighbor(Ctx& c) {
                                                                c.used.get());
     int S2 = 0;
     scan_exclusive(c, c.used.get(), c.newidx.get(), S, &S2);
-    c.site2.reserve(S2);
     c.sdesc2.reserve(S2);
     c.simg2.reserve(S2);
     c.scell2.reserve(S2);
-    k_compact_sites<<<grid_for(S, 256), 256, 0, c.stream>>>(c.used.get(), c.newidx.get(), S, c.site.get(),
-                                                            c.sdesc.get(), c.simg.get(), c.scell.get(),
-                                                            c.site2.get(), c.sdesc2.get(), c.simg2.get(),
-                                                            c.scell2.get());
+    k_compact_sites<<<grid_for(S, 256), 256, 0, c.stream>>>(c.used.get(), c.newidx.get(), S, c.sdesc.get(),
+                                                            c.simg.get(), c.scell.get(), c.sdesc2.get(),
+                                                            c.simg2.get(), c.scell2.get());
     k_remap_rows<<<grid_for(16L * P, 256), 256, 0, c.stream>>>(c.row_site.get(), c.nnum.get(), nl, P, c.cap,
                                                                 c.newidx.get());
-    c.site.swap(c.site2);
     c.sdesc.swap(c.sdesc2);
     c.simg.swap(c.simg2);
     c.scell.swap(c.scell2);
