@@ -286,6 +286,8 @@ struct Ctx {
   // into dense arrays in site (cell) order; the inner lists and the rows index them
   DBuf<int> live_idx, lsdesc, lsimg, lscell, row_live, d_nlive;
   DBuf<unsigned char> live_flag;  // one byte per site: the 4 MB of flags of a large model stay in L2
+  size_t live_flag_clean = 0;      // leading flags known to be zero (the compaction clears what it reads)
+  unsigned char* live_flag_at = nullptr;  // allocation those flags were cleared in
   DBuf<double4> site_live;        // FP64 records (x, y, z, F') of the live sites
   DBuf<float4> site32f;           // mixed path: FP32 cell-relative records of every site (for the prune)
   volatile int* h_steps = nullptr;  // host-mapped copy of the same, written by the check kernel
@@ -363,6 +365,7 @@ struct LiveOut {
   float4* site32;
   double4* site_live;
   int mixed;
+  unsigned char* clear;  // the flags, zeroed once read (ready for the next prune)
 };
 // idx = exclusive scan of the 0/1 flags, *d_total (device) = their sum, and the
 // flagged sites compacted into lo's dense arrays, in one pass after the sums

@@ -960,7 +960,13 @@ void build_inner(Ctx& c) {
   c.live_idx.reserve(c.S);
   c.row_live.reserve(c.P);
   c.d_nlive.reserve(1);
-  CAC_HIP(hipMemsetAsync(c.live_flag.get(), 0, c.S, c.stream));
+  // the last compaction left its flags zero; a new allocation or more sites
+  // than it read are cleared here
+  if (c.live_flag.get() != c.live_flag_at || c.live_flag_clean < (size_t)c.S) {
+    CAC_HIP(hipMemsetAsync(c.live_flag.get(), 0, c.S, c.stream));
+    c.live_flag_at = c.live_flag.get();
+  }
+  c.live_flag_clean = 0;  // until this prune's flags are compacted (and so cleared)
   const int blocks = (int)std::min<long>(grid_for(16L * c.P, 256), 8L * c.ncu);
   const unsigned* nl = reinterpret_cast<const unsigned*>(c.nlist.get());
   unsigned* il = reinterpret_cast<unsigned*>(c.ilist.get());

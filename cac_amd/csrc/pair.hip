@@ -276,8 +276,10 @@ void compact_live(Ctx& c) {
   c.live_idx.reserve(c.S);
   c.d_nlive.reserve(1);
   LiveOut lo{c.sdesc.get(), c.simg.get(), c.scell.get(), c.site.get(), c.site32f.get(), c.lsdesc.get(),
-             c.lsimg.get(), c.lscell.get(), c.site32.get(), c.site_live.get(), c.precision == 1 ? 1 : 0};
+             c.lsimg.get(), c.lscell.get(), c.site32.get(), c.site_live.get(), c.precision == 1 ? 1 : 0,
+             c.live_flag.get()};
   scan_live(c, c.live_flag.get(), c.live_idx.get(), c.S, c.d_nlive.get(), lo);
+  c.live_flag_clean = std::max(c.live_flag_clean, (size_t)c.S);
 }
 
 // ------------------------------------------------------------------ spline helpers

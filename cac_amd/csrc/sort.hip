@@ -144,6 +144,12 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const T* __restrict__ in
       if (base + j < n) out[base + j] = o[j];
   }
   if (MODE) {
+    // the flags this thread read are cleared for the next prune
+    if (base + SCAN_IPT <= n)
+      *reinterpret_cast<uint4*>(lo.clear + base) = make_uint4(0u, 0u, 0u, 0u);
+    else
+      for (int j = 0; j < SCAN_IPT; j++)
+        if (base + j < n) lo.clear[base + j] = 0;
     // each flagged site's dense index (-1 when unflagged), staged so that the
     // copies below run over consecutive sites across the workgroup
 #pragma unroll
