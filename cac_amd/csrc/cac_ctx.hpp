@@ -226,7 +226,7 @@ struct Ctx {
   DBuf<int> sdesc, simg, sdesc2, simg2, used, newidx;
   DBuf<int> tcell, row_point, row_site, ecount, eoffset, scan_tmp;
   DBuf<int> cell_start, occ, nearA, nearB, sort_keys;
-  DBuf<unsigned long long> tval, sort_vals;  // candidate site descriptor << 32 | wrap code, unsorted / sorted
+  DBuf<unsigned> tval, sort_vals;  // candidate site as (descriptor index) << 5 | wrap code, unsorted / sorted
   DBuf<unsigned long long> row_key;          // Morton keys of the rows' cells (and their sorted copy)
   DBuf<int> row_idx, row_tmp;
   DBuf<unsigned char> sort_tmp;
@@ -371,7 +371,7 @@ struct LiveOut {
 // flagged sites compacted into lo's dense arrays, in one pass after the sums
 void scan_live(Ctx& c, const unsigned char* flag, int* idx, int n, int* d_total, const LiveOut& lo);
 // stable sort of the candidate sites by cell: sort_keys / sort_vals, cell_start
-void sort_sites_by_cell(Ctx& c, const int* tcell, const unsigned long long* tval, int S);
+void sort_sites_by_cell(Ctx& c, const int* tcell, const unsigned* tval, int S);
 void site_positions(Ctx& c);  // FP64 records of every site from its descriptor (list build)
 void order_rows(Ctx& c);      // rows in Morton order of their cells
 

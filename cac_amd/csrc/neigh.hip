@@ -198,11 +198,20 @@ __global__ void k_point_occ(const double* __restrict__ x, int P, Grid g, int* __
   occ[cell] = 1;
 }
 
-__device__ __forceinline__ unsigned long long pack_site(int desc, int code) {
-  return ((unsigned long long)(unsigned)desc << 32) | (unsigned)code;
+// A candidate site packed in 32 bits for the cell sort: the descriptor as an
+// index (a point p as p, the interpolated atom -(1 + q) as P + q) times 32,
+// plus its wrap code (0..26); the build checks P + E nucell < 2^27.
+__device__ __forceinline__ unsigned pack_site(int desc, int code, int P) {
+  const unsigned d = desc >= 0 ? (unsigned)desc : (unsigned)P + (unsigned)(-desc - 1);
+  return (d << 5) | (unsigned)code;
 }
 
-__global__ void k_point_sites(const double* __restrict__ x, int P, Grid g, unsigned long long* __restrict__ tval,
+__device__ __forceinline__ int site_desc(unsigned v, int P) {
+  const int d = (int)(v >> 5);
+  return d < P ? d : -(d - P + 1);
+}
+
+__global__ void k_point_sites(const double* __restrict__ x, int P, Grid g, unsigned* __restrict__ tval,
                               int* __restrict__ tcell) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
@@ -213,7 +222,7 @@ __global__ void k_point_sites(const double* __restrict__ x, int P, Grid g, unsig
     cell = 0;
     code = 13;
   }
-  tval[p] = pack_site(p, code);
+  tval[p] = pack_site(p, code, P);
   tcell[p] = cell;
 }
 
@@ -247,7 +256,7 @@ __global__ __launch_bounds__(VB) void k_virt(const double* __restrict__ x, const
                                              const int* __restrict__ u2g, const int* __restrict__ nearmask, int A,
                                              int P, int nucell, Grid g, int* __restrict__ ecount,
                                              const int* __restrict__ eoffset,
-                                             unsigned long long* __restrict__ tval, int* __restrict__ tcell) {
+                                             unsigned* __restrict__ tval, int* __restrict__ tcell) {
   const int e = blockIdx.x;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -274,7 +283,7 @@ __global__ __launch_bounds__(VB) void k_virt(const double* __restrict__ x, const
     }
     if (FILL && need) {
       const int s = base + total + woff + rank;
-      tval[s] = pack_site(-(1 + e * nucell + u), code);
+      tval[s] = pack_site(-(1 + e * nucell + u), code, P);
       tcell[s] = cell;
     }
     total += chunk;
@@ -284,17 +293,17 @@ __global__ __launch_bounds__(VB) void k_virt(const double* __restrict__ x, const
 }
 
 // sorted (cell, descriptor) pairs -> site descriptors, packed cell coordinates, row flags
-__global__ void k_unpack_sites(const int* __restrict__ key, const unsigned long long* __restrict__ val, int S, Grid g,
+__global__ void k_unpack_sites(const int* __restrict__ key, const unsigned* __restrict__ val, int S, int P, Grid g,
                                int* __restrict__ sdesc, int* __restrict__ simg, int* __restrict__ scell,
                                int* __restrict__ isrow) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S) return;
   const int cid = key[k];
-  const unsigned long long v = val[k];
-  const int desc = (int)(unsigned)(v >> 32);
+  const unsigned v = val[k];
+  const int desc = site_desc(v, P);
   scell[k] = cell_pack(cid % g.nc[0], (cid / g.nc[0]) % g.nc[1], cid / (g.nc[0] * g.nc[1]));
   sdesc[k] = desc;
-  simg[k] = (int)(unsigned)(v & 0xffffffffull);
+  simg[k] = (int)(v & 31u);
   isrow[k] = desc >= 0;
 }
 
@@ -632,6 +641,9 @@ void build_neighbor(Ctx& c) {
   }
   const int S = P + nvirt;
   if ((long)S >= (1L << IMG_SHIFT)) throw HipError{hipErrorInvalidValue, "more than 2^27 sites"};
+  // the cell sort packs a candidate's descriptor index in 27 bits (pack_site)
+  if ((long)P + (long)c.E * (long)c.nucell >= (1L << 27))
+    throw HipError{hipErrorInvalidValue, "more than 2^27 atoms, Gauss points and interpolated atoms"};
   c.tval.reserve(S);
   c.tcell.reserve(S);
   k_point_sites<<<grid_for(P, 256), 256, 0, c.stream>>>(c.x.get(), P, g, c.tval.get(), c.tcell.get());
@@ -649,7 +661,7 @@ void build_neighbor(Ctx& c) {
   c.row_point.reserve(P);
   c.row_site.reserve(P);
   c.scell.reserve(S);
-  k_unpack_sites<<<grid_for(S, 256), 256, 0, c.stream>>>(c.sort_keys.get(), c.sort_vals.get(), S, g, c.sdesc.get(),
+  k_unpack_sites<<<grid_for(S, 256), 256, 0, c.stream>>>(c.sort_keys.get(), c.sort_vals.get(), S, P, g, c.sdesc.get(),
                                                          c.simg.get(), c.scell.get(), c.isflag.get());
   c.S = S;
   site_positions(c);

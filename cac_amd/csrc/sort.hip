@@ -234,7 +234,7 @@ __global__ void k_cell_bounds(const int* __restrict__ key, int S, long ncells, i
   for (long c = prev + 1; c <= cur; c++) cell_start[c] = (int)i;
 }
 
-void sort_sites_by_cell(Ctx& c, const int* tcell, const unsigned long long* tval, int S) {
+void sort_sites_by_cell(Ctx& c, const int* tcell, const unsigned* tval, int S) {
   int bits = 1;
   while (bits < 31 && (1L << bits) < c.ncells) bits++;
   c.sort_keys.reserve(S);
