@@ -162,6 +162,8 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, U
   if (g.skip()) return;
   const bool full = all || (g.step >= 0 && g.prune());
   const int n = full ? S : *nlive;
+  // the grid covers every site; past the live ones a dense refresh has no work
+  if (blockIdx.x * blockDim.x * RSPT >= n) return;
   const int* dsc = full ? sdesc : lsdesc;
   const int* img = full ? simg : lsimg;
   const int k0 = blockIdx.x * blockDim.x * RSPT + threadIdx.x;
@@ -507,6 +509,7 @@ __device__ __forceinline__ void embed_block(int blk, const double* __restrict__ 
                                             double4* __restrict__ site_live, float4* __restrict__ site32) {
 #pragma clang fp contract(off)
   const int n = *nlive;
+  if (blk * blockDim.x * RSPT >= n) return;  // the grid covers every site, not only the live ones
   const int k0 = blk * blockDim.x * RSPT + threadIdx.x;
   int kk[RSPT], d[RSPT];
 #pragma unroll
