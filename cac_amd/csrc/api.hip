@@ -34,12 +34,19 @@ Ctx::~Ctx() {
 
 // Timing events record with a device-scope release: a system-scope one (the
 // default) adds its cache write-back to the interval it closes.
-static hipEvent_t take_event(Ctx& c) {
-  if (c.ev_used == c.ev_pool.size()) {
+// pending timed sections before their times are read back; each holds two events
+constexpr size_t MAX_PENDING = 512;
+
+static void grow_events(Ctx& c, size_t n) {
+  while (c.ev_pool.size() < n) {
     hipEvent_t e = nullptr;
     CAC_HIP(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
     c.ev_pool.push_back(e);
   }
+}
+
+static hipEvent_t take_event(Ctx& c) {
+  if (c.ev_used == c.ev_pool.size()) grow_events(c, c.ev_used + 1);
   return c.ev_pool[c.ev_used++];
 }
 
@@ -67,7 +74,7 @@ static bool timed_section(const Ctx& c, const char* nm) {
 
 ScopedTimer::ScopedTimer(Ctx& cc, const char* nm, bool kernel_bound) : c(cc), name(nm), bound(kernel_bound) {
   if (!timed_section(c, nm)) return;
-  if (c.ev_pending.size() >= 512) resolve_timers(c);
+  if (c.ev_pending.size() >= MAX_PENDING) resolve_timers(c);
   start = take_event(c);
   stop = take_event(c);
   if (!bound) CAC_HIP(hipEventRecord(start, c.stream));
@@ -1084,6 +1091,9 @@ int cac_hip_enable_timing(cac_hip_ctx* h, int on) {
     resolve_timers(c);
     c.timing = on;
     c.timers.clear();
+    // every event the timed sections will take before a read-back, created
+    // here: a creation costs host time that a host-bound step would show
+    if (on) grow_events(c, 2 * MAX_PENDING);
     return CAC_HIP_OK;
   });
 }
