@@ -42,8 +42,11 @@ constexpr int ROW_PAD = 16;
 constexpr int IMG_SHIFT = 27;
 constexpr unsigned SITE_MASK = (1u << IMG_SHIFT) - 1u;  // list entry = site | image << 27, read as unsigned
 
-// a site's neighbor cell is kept as packed coordinates ix | iy << 11 | iz << 22
-constexpr int CELL_BITS_X = 11, CELL_BITS_Y = 11, CELL_BITS_Z = 10;
+// a site's neighbor cell is kept as packed coordinates ix | iy << 10 | iz << 21;
+// the top bit stays clear (the mixed path's full records set it for a site
+// that is a point, POINT_SITE_BIT)
+constexpr int CELL_BITS_X = 10, CELL_BITS_Y = 11, CELL_BITS_Z = 10;
+constexpr unsigned POINT_SITE_BIT = 1u << 31;
 __host__ __device__ inline int cell_pack(int ix, int iy, int iz) {
   return ix | (iy << CELL_BITS_X) | (iz << (CELL_BITS_X + CELL_BITS_Y));
 }

@@ -894,10 +894,11 @@ __global__ __launch_bounds__(256) void k_prune(const double4* __restrict__ site,
         const unsigned long long m = __ballot(keep);
         const unsigned gm = (unsigned)((m >> gbase) & 0xFFFFull);
         const int rank = __popc(gm & ((1u << lane) - 1u));
-        if (keep) {
-          out[count + rank] = eout;
-          live[ent[u] & SITE_MASK] = 1;
-        }
+        if (keep) out[count + rank] = eout;
+        // a point's site is flagged by the point's own row (above); only the
+        // other sites are flagged here, which in the mixed path skips the
+        // scattered byte stores of atom-atom pairs
+        if (keep && !(MIXED && (__float_as_uint(sj32[u].w) & POINT_SITE_BIT))) live[ent[u] & SITE_MASK] = 1;
         count += __popc(gm);
       }
     }

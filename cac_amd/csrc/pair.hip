@@ -200,12 +200,14 @@ __global__ __launch_bounds__(256) void k_refresh(const double* __restrict__ x, U
     for (int c = 0; c < 3; c++) y[c] = y[c] + sh[c];
     const int k = kk[t];
     if (MIXED) {
-      // the full records carry the site's packed cell in w (the prune reads
-      // it there); the dense ones get F'(rho) in w from the embedding pass
+      // the full records carry the site's packed cell in w, with the top bit
+      // set for a site that is a point (the prune reads both there); the
+      // dense ones get F'(rho) in w from the embedding pass
+      const unsigned wfull = (unsigned)cc[t] | (d[t] >= 0 ? POINT_SITE_BIT : 0u);
       const float4 r32 = make_float4((float)(y[0] - (fr.lo[0] + cell_ix(cc[t]) * fr.cs[0])),
                                      (float)(y[1] - (fr.lo[1] + cell_iy(cc[t]) * fr.cs[1])),
                                      (float)(y[2] - (fr.lo[2] + cell_iz(cc[t]) * fr.cs[2])),
-                                     full ? __int_as_float(cc[t]) : 0.0f);
+                                     full ? __uint_as_float(wfull) : 0.0f);
       (full ? site32f : site32)[k] = r32;
     } else {
       (full ? site : site_live)[k] = make_double4(y[0], y[1], y[2], 0.0);
