@@ -230,8 +230,10 @@ struct Ctx {
   DBuf<int> tcell, row_point, row_site, ecount, eoffset, scan_tmp;
   DBuf<int> cell_start, occ, nearA, nearB, sort_keys;
   DBuf<unsigned> tval, sort_vals;  // candidate site as (descriptor index) << 5 | wrap code, unsorted / sorted
-  DBuf<unsigned long long> row_key;          // Morton keys of the rows' cells (and their sorted copy)
+  DBuf<unsigned long long> row_key;          // Morton keys of the cells (and their sorted copy)
   DBuf<int> row_idx, row_tmp;
+  DBuf<int> cell_morton, morton_rows;  // cells in Morton order (per grid shape); rows per cell and their starts
+  int morton_nc[3] = {0, 0, 0};        // the grid shape cell_morton was sorted for
   DBuf<unsigned char> sort_tmp;
   DBuf<int> nnum, nlist;
   DBuf<int> nearrow;  // per row: a candidate lies within the near-cutoff band

@@ -253,6 +253,12 @@ void sort_sites_by_cell(Ctx& c, const int* tcell, const unsigned* tval, int S) {
 // Rows (points) in Morton order of their cells: consecutive rows, and so the
 // rows one XCD works on at a time, form compact 3D blocks, whose neighbor sites
 // fit the XCD's L2 far better than the slabs of the plain cell order.
+//
+// The rows arrive in site order, so each cell's rows are one contiguous run in
+// cell order.  Reordering the runs by the Morton order of their cells (a
+// permutation of the cells that depends only on the grid shape, sorted once
+// per shape) gives the rows a stable sort by Morton key would give: a scan of
+// the runs' lengths in that order, then one copy per run.
 __device__ __forceinline__ unsigned long long spread3(unsigned v) {
   unsigned long long x = v & 0x1fffffull;
   x = (x | (x << 32)) & 0x1f00000000ffffull;
@@ -263,50 +269,86 @@ __device__ __forceinline__ unsigned long long spread3(unsigned v) {
   return x;
 }
 
-__global__ void k_row_keys(const int* __restrict__ row_site, const int* __restrict__ scell, int R,
-                           unsigned long long* __restrict__ key, int* __restrict__ idx) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R) return;
-  const int cc = scell[row_site[r]];
-  key[r] = spread3((unsigned)cell_ix(cc)) | (spread3((unsigned)cell_iy(cc)) << 1) |
-           (spread3((unsigned)cell_iz(cc)) << 2);
-  idx[r] = r;
+__global__ void k_cell_morton_keys(int ncells, int nc0, int nc1, unsigned long long* __restrict__ key,
+                                   int* __restrict__ idx) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const unsigned ix = (unsigned)(c % nc0), iy = (unsigned)((c / nc0) % nc1), iz = (unsigned)(c / (nc0 * nc1));
+  key[c] = spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
+  idx[c] = c;
 }
 
-__global__ void k_row_gather(const int* __restrict__ perm, int R, const int* __restrict__ rs_in,
-                             const int* __restrict__ rp_in, int* __restrict__ rs_out, int* __restrict__ rp_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= R) return;
-  const int r = perm[i];
-  rs_out[i] = rs_in[r];
-  rp_out[i] = rp_in[r];
+// the run of cell c: rows [first, first + count), clamped to the P rows
+__device__ __forceinline__ void row_run(const int* __restrict__ cell_start, const int* __restrict__ rowidx, int c,
+                                        int S, int P, int& first, int& count) {
+  const int a = cell_start[c], b = cell_start[c + 1];
+  const int ra = min(a < S ? rowidx[a] : P, P), rb = min(b < S ? rowidx[b] : P, P);
+  first = ra;
+  count = max(rb - ra, 0);
 }
 
-void order_rows(Ctx& c) {
-  const int R = c.P;
-  c.row_key.reserve(2 * (size_t)R);
-  c.row_idx.reserve(2 * (size_t)R);
+__global__ void k_row_runs(const int* __restrict__ morton, const int* __restrict__ cell_start,
+                           const int* __restrict__ rowidx, int ncells, int S, int P, int* __restrict__ count) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= ncells) return;
+  int first, n;
+  row_run(cell_start, rowidx, morton[m], S, P, first, n);
+  count[m] = n;
+}
+
+__global__ void k_row_copy(const int* __restrict__ morton, const int* __restrict__ cell_start,
+                           const int* __restrict__ rowidx, const int* __restrict__ start, int ncells, int S, int P,
+                           const int* __restrict__ rs_in, const int* __restrict__ rp_in, int* __restrict__ rs_out,
+                           int* __restrict__ rp_out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= ncells) return;
+  int first, n;
+  row_run(cell_start, rowidx, morton[m], S, P, first, n);
+  const int o = start[m];
+  for (int j = 0; j < n && o + j < P; j++) {
+    rs_out[o + j] = rs_in[first + j];
+    rp_out[o + j] = rp_in[first + j];
+  }
+}
+
+static void sort_cells_morton(Ctx& c) {
+  const int nc = (int)c.ncells;
+  if (c.morton_nc[0] == c.nc[0] && c.morton_nc[1] == c.nc[1] && c.morton_nc[2] == c.nc[2]) return;
+  c.row_key.reserve(2 * (size_t)nc);
+  c.row_idx.reserve(2 * (size_t)nc);
   unsigned long long* key = c.row_key.get();
   int* idx = c.row_idx.get();
-  k_row_keys<<<grid_for(R, 256), 256, 0, c.stream>>>(c.row_site.get(), c.scell.get(), R, key, idx);
+  k_cell_morton_keys<<<grid_for(nc, 256), 256, 0, c.stream>>>(nc, c.nc[0], c.nc[1], key, idx);
   // the Morton key holds 3 bits per bit of the largest cell index
   int b = 1;
   while (b < 21 && (1L << b) < std::max(c.nc[0], std::max(c.nc[1], c.nc[2]))) b++;
   const int bits = 3 * b;
-  // onesweep LSD passes over the key's bits at every size: below a million
-  // items the library would otherwise merge-sort, in about 20 launches
-  using OnesweepOnly = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                                  rocprim::default_config, 0>;
   size_t tmp = 0;
-  CAC_HIP(rocprim::radix_sort_pairs<OnesweepOnly>(nullptr, tmp, key, key + R, idx, idx + R, (unsigned)R, 0u,
-                                                  (unsigned)bits, c.stream));
+  CAC_HIP(rocprim::radix_sort_pairs(nullptr, tmp, key, key + nc, idx, idx + nc, (unsigned)nc, 0u, (unsigned)bits,
+                                    c.stream));
   c.sort_tmp.reserve(tmp + 16);
-  CAC_HIP(rocprim::radix_sort_pairs<OnesweepOnly>(c.sort_tmp.get(), tmp, key, key + R, idx, idx + R, (unsigned)R,
-                                                  0u, (unsigned)bits, c.stream));
+  CAC_HIP(rocprim::radix_sort_pairs(c.sort_tmp.get(), tmp, key, key + nc, idx, idx + nc, (unsigned)nc, 0u,
+                                    (unsigned)bits, c.stream));
+  c.cell_morton.reserve(nc);
+  CAC_HIP(hipMemcpyAsync(c.cell_morton.get(), idx + nc, sizeof(int) * nc, hipMemcpyDeviceToDevice, c.stream));
+  for (int d = 0; d < 3; d++) c.morton_nc[d] = c.nc[d];
+}
+
+void order_rows(Ctx& c) {
+  const int R = c.P, nc = (int)c.ncells;
+  sort_cells_morton(c);
+  c.morton_rows.reserve(2 * (size_t)nc);
+  int* count = c.morton_rows.get();
+  int* start = count + nc;
+  k_row_runs<<<grid_for(nc, 256), 256, 0, c.stream>>>(c.cell_morton.get(), c.cell_start.get(), c.rowidx.get(), nc,
+                                                      c.S, R, count);
+  scan_launch(c, count, start, nc, nullptr, nullptr);
   c.row_tmp.reserve(2 * (size_t)R);
   int* rs = c.row_tmp.get();
   int* rp = rs + R;
-  k_row_gather<<<grid_for(R, 256), 256, 0, c.stream>>>(idx + R, R, c.row_site.get(), c.row_point.get(), rs, rp);
+  k_row_copy<<<grid_for(nc, 256), 256, 0, c.stream>>>(c.cell_morton.get(), c.cell_start.get(), c.rowidx.get(),
+                                                      start, nc, c.S, R, c.row_site.get(), c.row_point.get(), rs,
+                                                      rp);
   CAC_HIP(hipMemcpyAsync(c.row_site.get(), rs, sizeof(int) * R, hipMemcpyDeviceToDevice, c.stream));
   CAC_HIP(hipMemcpyAsync(c.row_point.get(), rp, sizeof(int) * R, hipMemcpyDeviceToDevice, c.stream));
 }
